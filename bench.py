@@ -1,0 +1,235 @@
+"""Throughput of the DEC GPE step on one MI355X, FP32 form vs FP64 form.
+
+Workload (default, spin-1 BCC, the north-star case): 128^3 BCC blocks (12 unknowns each,
+25,165,824 unknowns; 4,194,304 lattice points), box 24^3, trap (1, 1, 166/126), c0 = 1e4,
+c1 = +500 (polar), constant quadratic Zeeman q = 0.1, field b'(x, y, -2z) + Bz(t) z with
+b' = 4e-3 and Bz piecewise linear through 16 seeded knots U[-0.1, 0.1].  Initial state:
+polar Thomas-Fermi profile relaxed by 1000 imaginary-time steps, plus seeded 0.5% noise.
+Then a forward-Euler start, W warm-up steps and K timed 3-level leapfrog steps at dt = 1e-4,
+in FP32 (FP64 reductions) and in FP64; the two final states are compared.
+
+`--workload scalar` runs the scalar staggered case instead: 192^3 BCC blocks, box 9.6^3,
+g = 300, vortex-ring imprint, dt = 1e-4.
+
+One step = one sweep over all unknowns (3-level) or one odd+even pair (staggered); a
+node-update = one unknown advanced by one sweep.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "node-updates/s on 1 GPU, FP32-form vs FP64 at ≤1e-4 rel L2; % HBM roofline"
+PEAK_HBM_GBS = 8000.0
+
+
+def _sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def workload_spin1(seed=2024):
+    from spacetime_amd import gpe
+    cells = (128, 128, 128)
+    box = 24.0
+    cfg = gpe.Config.centred_box("bcc", cells, box / cells[0], ncomp=3, dt=1e-4, trap=(1.0, 1.0, 166.0 / 126.0),
+                                 c0=1.0e4, c1=500.0, q=0.1)
+    rng = np.random.default_rng(seed)
+    knots = rng.uniform(-0.1, 0.1, 16)
+    info = dict(workload="spin1_bcc_128^3", lattice="bcc", cells=list(cells), box=box, components=3,
+                scheme="leapfrog3", dt=cfg.dt, c0=cfg.c0, c1=cfg.c1, q=cfg.q, bq=4e-3, bz_knots=16, seed=seed)
+    return cfg, knots, info
+
+
+def workload_scalar():
+    from spacetime_amd import gpe
+    cells = (192, 192, 192)
+    box = 9.6
+    cfg = gpe.Config.centred_box("bcc", cells, box / cells[0], ncomp=1, dt=1e-4, scheme=gpe.STAGGERED, c0=300.0)
+    info = dict(workload="scalar_bcc_192^3", lattice="bcc", cells=list(cells), box=box, components=1,
+                scheme="staggered", dt=cfg.dt, g=cfg.c0, imprint="vortex ring R=2")
+    return cfg, None, info
+
+
+def prepare_state(cfg, knots, itp_steps, itp_dt, log):
+    """FP64 ground state by imaginary time on the GPU, then the imprint / noise."""
+    from spacetime_amd import gpe, states
+    import dataclasses
+    r = states.positions(cfg)
+    tf, mu = states.thomas_fermi(cfg, cfg.c0, r=r)
+    psi = np.zeros(cfg.shape, dtype=np.complex128)
+    psi[cfg.ncomp // 2] = tf
+    del tf
+    cfg64 = dataclasses.replace(cfg, precision=gpe.FP64, scheme=gpe.LEAPFROG3)
+    t0 = time.time()
+    with gpe.Stepper(cfg64) as st:
+        st.set_state(psi)
+        st.itp(itp_steps, itp_dt)
+        psi = st.get_state(0)
+        e = st.observables(0)
+    log("itp: %d steps dt=%g in %.2fs, E=%.6f mu_TF=%.4f" % (itp_steps, itp_dt, time.time() - t0, e.energy, mu))
+    if cfg.ncomp == 1:
+        psi = psi * states.vortex_ring_phase(r, 2.0)[None]
+        psi = states.multiplicative_noise(psi, 0.01, 54363)
+    else:
+        psi = states.project(psi, "polar")
+        psi = states.spinor_noise(psi, 0.005, 54363)
+    psi *= np.sqrt(1.0 / states.norm(cfg, psi))
+    return psi
+
+
+def run_form(cfg, psi0, knots, warmup, steps, log):
+    from spacetime_amd import gpe, states
+    with gpe.Stepper(cfg) as st:
+        if knots is not None:
+            st.set_ramp(q_segments=[(4e-3, 1e30, 0)], b_segments=states.knot_ramp(knots, 2.0))
+        st.set_state(psi0)
+        o0 = st.observables(0)
+        st.start(gpe.START_EULER)
+        if warmup:
+            st.step(warmup)
+        st.sync()
+        _sync()
+        t0 = time.perf_counter()
+        st.step(steps)
+        st.sync()
+        _sync()
+        wall = time.perf_counter() - t0
+        ms_sweep, bytes_sweep = st.last_step_timing()
+        o1 = st.observables(0)
+        psi = st.get_state(0)
+    per = 2 if cfg.scheme == gpe.STAGGERED else 1
+    nodes = np.prod(cfg.dims) * cfg.V
+    res = dict(wall_s=wall, ms_per_step=1e3 * wall / steps, kernel_ms_per_sweep=ms_sweep,
+               bytes_per_sweep=bytes_sweep, node_updates_per_s=nodes * steps * per / wall,
+               kernel_node_updates_per_s=nodes / (ms_sweep * 1e-3),
+               achieved_gbs=bytes_sweep / (ms_sweep * 1e-3) / 1e9,
+               norm_drift=abs(o1.norm - o0.norm) / o0.norm, energy_drift=abs(o1.energy - o0.energy) / abs(o0.energy),
+               energy=o1.energy, norm=o1.norm)
+    log("%s: %d steps in %.3fs, kernel %.4f ms/sweep, %.3e node-updates/s, %.1f GB/s" % (
+        "fp64" if cfg.precision else "fp32", steps, wall, ms_sweep, res["node_updates_per_s"], res["achieved_gbs"]))
+    return res, psi
+
+
+def cpu_baseline(cfg, psi0, log, budget_s=15.0):
+    """FP64 oracle (C restatement of the reference update) on the host cores, same grid."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle
+    from helpers import oracle_params, with_field
+    from spacetime_amd import lattice
+    o = Oracle()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    threads = o.set_threads(threads)
+    L = o.lattice(lattice.get(cfg.lattice))
+    p = oracle_params(o, cfg)
+    if cfg.ncomp > 1:
+        with_field(p, bq=4e-3, bb=(0.0, 0.0, 0.0))
+    a = np.ascontiguousarray(psi0)
+    b = a.copy()
+    coef = cfg.dt if cfg.scheme == 1 else 2 * cfg.dt
+    t0 = time.perf_counter()
+    o.update(L, p, b, a, coef)
+    t1 = time.perf_counter() - t0
+    n = max(1, min(50, int(budget_s / max(t1, 1e-3))))
+    t0 = time.perf_counter()
+    for k in range(n):
+        o.update(L, p, b, a, coef)
+        a, b = b, a
+    el = time.perf_counter() - t0
+    nodes = np.prod(cfg.dims) * cfg.V
+    val = nodes * n / el
+    log("cpu oracle: %d sweeps of %d unknowns in %.2fs on %d threads: %.3e node-updates/s" % (n, nodes, el, threads,
+                                                                                          val))
+    return dict(value=val, unit="node-updates/s", cores=threads, kind="port",
+                sample="%d FP64 sweeps of the full %s grid (%d unknowns) with the C oracle" % (n, "x".join(
+                    str(d) for d in cfg.dims), nodes))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", choices=["spin1", "scalar"], default="spin1")
+    ap.add_argument("--itp-steps", type=int, default=1000)
+    ap.add_argument("--itp-dt", type=float, default=2e-4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--quiet", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1).",
+              file=sys.stderr)
+        sys.exit(2)
+    log = (lambda *a: None) if args.quiet else (lambda m: print("[bench] " + m, file=sys.stderr, flush=True))
+
+    from spacetime_amd import gpe
+    import dataclasses
+    cfg, knots, info = workload_spin1() if args.workload == "spin1" else workload_scalar()
+    nodes = int(np.prod(cfg.dims) * cfg.V)
+    info.update(unknowns=nodes, lattice_points=int(2 * np.prod(cfg.dims)))
+    log("workload %s: %d unknowns" % (info["workload"], nodes))
+    psi0 = prepare_state(cfg, knots, args.itp_steps, args.itp_dt, log)
+
+    r32, psi32 = run_form(dataclasses.replace(cfg, precision=gpe.FP32), psi0, knots, args.warmup, args.steps, log)
+    acc = {}
+    r64 = None
+    if not args.no_fp64:
+        r64, psi64 = run_form(dataclasses.replace(cfg, precision=gpe.FP64), psi0, knots, args.warmup, args.steps,
+                              log)
+        acc["rel_l2_fp32_vs_fp64"] = float(np.linalg.norm((psi32 - psi64).ravel()) / np.linalg.norm(psi64.ravel()))
+        del psi64
+    acc.update(norm_drift_fp32=r32["norm_drift"], energy_drift_fp32=r32["energy_drift"])
+    if r64:
+        acc.update(norm_drift_fp64=r64["norm_drift"], energy_drift_fp64=r64["energy_drift"])
+        acc["speedup_fp32_over_fp64"] = r32["node_updates_per_s"] / r64["node_updates_per_s"]
+    del psi32
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(cfg, psi0, log)
+
+    line = {
+        "metric": METRIC,
+        "value": r32["node_updates_per_s"],
+        "unit": "node-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": info,
+        "roofline": {
+            "bound": "hbm",
+            "achieved": r32["achieved_gbs"],
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": r32["achieved_gbs"] / PEAK_HBM_GBS,
+            "traffic": None,
+            "kernel_ms_per_sweep": r32["kernel_ms_per_sweep"],
+            "algorithmic_bytes_per_sweep": r32["bytes_per_sweep"],
+        },
+        "cpu_baseline": cpu,
+        "fp64_form": None if r64 is None else {
+            "value": r64["node_updates_per_s"], "ms_per_step": r64["ms_per_step"],
+            "achieved_gbs": r64["achieved_gbs"], "frac": r64["achieved_gbs"] / PEAK_HBM_GBS},
+        "accuracy": acc,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,164 @@
+/*
+ * gpe.h -- C ABI of the MI355X DEC Gross-Pitaevskii stepper (libgpe.so).
+ *
+ * The reference has no library boundary: each solver is an executable whose host loop
+ * (integrateInTime) allocates pitched buffers and launches one kernel per sweep.  These
+ * entry points replace that loop piece by piece; each declaration names the reference
+ * code it stands in for.  All calls return 0 on success or a negative GPE_E* code, never
+ * exit or abort; gpe_last_error() describes the last failure of a context.
+ *
+ * State layout across the ABI (host or device arrays, complex interleaved re/im):
+ *   psi[2 * (((c * nz + z) * ny + y) * nx + x) * V + v) + {0, 1}]
+ * component c (m = F, F-1, ..., -F), block (x, y, z), unknown v of the block.  The device
+ * keeps its own layout; the ABI layout is independent of it.
+ *
+ * Threading: a context owns one HIP stream; calls on one context are stream-ordered and
+ * not thread-safe; distinct contexts may be driven from distinct threads.  Calls that
+ * return host data (get_state, observables) synchronise the stream.
+ */
+#ifndef GPE_H
+#define GPE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define GPE_OK 0
+#define GPE_EINVAL (-22)  /* bad argument (lattice id, sizes, pointers)          */
+#define GPE_ENOMEM (-12)  /* device or host allocation failed                     */
+#define GPE_EDEVICE (-5)  /* HIP runtime error                                    */
+#define GPE_ESTATE (-71)  /* call not valid in the current state (e.g. no start)  */
+#define GPE_ERANGE (-34)  /* non-finite value detected                            */
+
+/* lattices: the unit-cell tables of the reference's mesh.h files */
+#define GPE_LATTICE_CUBE 0 /* Spin1GpeCube/mesh.h            1 unknown / block  */
+#define GPE_LATTICE_BCC 1  /* Spin1GpeBCC/mesh.h            12 unknowns / block */
+#define GPE_LATTICE_FCC 2  /* GrossPitaevskiiGpuFcc/mesh.h  12 unknowns / block */
+#define GPE_LATTICE_A15 3  /* GrossPitaevskiiGpuA15/mesh.h  46 unknowns / block */
+#define GPE_LATTICE_C15 4  /* GrossPitaevskiiGpuC15/mesh.h 136 unknowns / block */
+
+#define GPE_FP32 0 /* FP32 state and stencil arithmetic, FP64 reductions */
+#define GPE_FP64 1 /* FP64 throughout                                    */
+
+/* time integrators */
+#define GPE_SCHEME_LEAPFROG3 0 /* psi_{k+1} = psi_{k-1} - 2i dt H psi_k   (Spin1GpeBCC.cu:914-916)  */
+#define GPE_SCHEME_STAGGERED 1 /* odd += -i dt H(even); even += -i dt H(odd) (GPBlocks.cpp:560-606) */
+
+/* how the second time level is made before stepping */
+#define GPE_START_EULER 0 /* psi_1 = psi_0 - i dt H psi_0 (Spin1GpeBCC.cu:788-790)            */
+#define GPE_START_PHASE 1 /* psi_{-1/2} = exp(+i mu dt/2) psi_0 (GPBlocks.cpp:516-523)       */
+#define GPE_START_COPY 2  /* both levels = psi_0 (GPUGrossPitaevskii_BCC.cu:589-590)        */
+
+typedef struct gpe_ctx gpe_ctx;
+
+/* Physical and numerical set-up of one run.  Replaces the compile-time constants and the
+ * integrateInTime arguments of the reference solvers (e.g. Spin1GpeBCC.cu:34-110,1105-1112). */
+typedef struct {
+    int lattice;          /* GPE_LATTICE_*                                             */
+    int ncomp;            /* 1 (scalar), 3 (spin-1), 5 (spin-2)                        */
+    int precision;        /* GPE_FP32 / GPE_FP64                                       */
+    int scheme;           /* GPE_SCHEME_*                                              */
+    int nx, ny, nz;       /* blocks per axis                                           */
+    double block_scale;   /* physical length of one table unit                        */
+    double p0[3];         /* physical position of the origin of block (0,0,0)          */
+    double dt;            /* time step (3-level: between levels; staggered: full step) */
+    double trap[3];       /* V = 0.5 sum_a (trap_a r_a)^2 + trap_offset               */
+    double trap_offset;   /* the reference adds 100 (Spin1GpeBCC.cu:121)              */
+    double trap_off_time; /* V = 0 for t >= trap_off_time (Spin1GpeBCC.cu:114-116)     */
+    double c0;            /* density coupling (g for the scalar)                       */
+    double c1;            /* spin exchange (reference "c2")                            */
+    double c2;            /* spin-2 singlet (reference "c4")                           */
+    double alpha;         /* three-body loss: H += -i alpha n^2 (Spin1GpeBCC.cu:876-880) */
+    double quad_sign;     /* H += quad_sign (Bquad . f)^2; reference +1 spin-1, -1 spin-2 */
+    double q;             /* constant quadratic Zeeman H += q f_z^2                    */
+    double center[3];     /* quadrupole field centre (Spin1GpeBCC.cu:124-137)          */
+} gpe_config;
+
+/* Piecewise field ramp (getSignal, Spin1GpeBCC/AliceRingRamps.h:74-140 and
+ * Spin2GpeBCC/KnotRamps.h:49-116).  Segment type: 0 constant, 1 linear from the previous
+ * value, 2 sqrt(t/T), 3 1-exp(-t/tau).  Past the last segment the last value is held.
+ *   B(r)     = scale_q  * Bq * (x-cx, y-cy, -2(z-cz)) + scale_b  * Bb
+ *   Bquad(r) = qscale_q * Bq * (x-cx, y-cy, -2(z-cz)) + qscale_b * Bb
+ * evaluated at ramp time = time_scale * t (Spin1GpeBCC.cu:1577-1587). */
+typedef struct {
+    int nq;
+    const double *q_val, *q_dur;
+    const int *q_type;
+    int nb;
+    const double *b_val; /* nb * 3 */
+    const double *b_dur;
+    const int *b_type;
+    double tau, time_scale;
+    double scale_q, scale_b, qscale_q, qscale_b;
+} gpe_ramp;
+
+/* FP64 observables of one time level (Spin1GpeBCC.cu:191-271,955-1020). */
+typedef struct {
+    double norm;           /* sum dv |psi|^2                                   */
+    double energy;         /* GP energy functional E[psi]                       */
+    double expect_h;       /* Re <psi|H psi>: the reference's "energy" (F5)     */
+    double magnetization[3]; /* sum dv F                                       */
+    double max_density;    /* max |psi|^2                                       */
+    double time;           /* simulation time of this level                    */
+    int finite;            /* 0 if any non-finite value was seen                */
+} gpe_observables_t;
+
+/* Create a context: allocates both time levels on the current HIP device and uploads
+ * the lattice tables.  Replaces integrateInTime's set-up (Spin1GpeBCC.cu:1105-1172). */
+int gpe_create(const gpe_config *cfg, gpe_ctx **out);
+int gpe_destroy(gpe_ctx *ctx);
+const char *gpe_last_error(const gpe_ctx *ctx);
+
+/* Number of unknowns per block for a lattice id (VALUES_IN_BLOCK), or GPE_EINVAL. */
+int gpe_lattice_values(int lattice);
+
+/* Upload / download one time level: level 0 = current psi_k, level 1 = previous psi_{k-1}.
+ * Host pointers, complex128, ABI layout.  Replaces cudaMemcpy3D (Spin1GpeBCC.cu:1344-1345,
+ * 1617,1646).  set_state(level 0) also sets the time of the run to t. */
+int gpe_set_state(gpe_ctx *ctx, int level, const double *psi, double t);
+int gpe_get_state(gpe_ctx *ctx, int level, double *psi);
+/* Same with device pointers (complex128, ABI layout) on the context's device. */
+int gpe_set_state_device(gpe_ctx *ctx, int level, const double *dpsi, double t);
+int gpe_get_state_device(gpe_ctx *ctx, int level, double *dpsi);
+
+/* Static linear and quadratic Zeeman fields (MagFields Bs, Spin1GpeBCC.cu:1429-1432). */
+int gpe_set_field(gpe_ctx *ctx, double bq, const double bb[3], double bq_quad, const double bb_quad[3]);
+/* Time-dependent fields; replaces the per-launch getSignal (Spin1GpeBCC.cu:1583-1587). */
+int gpe_set_ramp(gpe_ctx *ctx, const gpe_ramp *ramp);
+
+/* Make the second time level from level 0 (GPE_START_*; mu used by GPE_START_PHASE).
+ * GPE_START_EULER replaces forwardEuler (Spin1GpeBCC.cu:667-791, launch :1424-1434). */
+int gpe_start(gpe_ctx *ctx, int mode, double mu);
+
+/* Advance n time steps (3-level: n sweeps; staggered: 2n sweeps).  Replaces the leapfrog
+ * launches of the host loop (Spin1GpeBCC.cu:1571-1603, GPUGrossPitaevskii_BCC.cu:883-889). */
+int gpe_step(gpe_ctx *ctx, int nsteps);
+
+/* n imaginary-time steps psi <- N(psi - dt_itp H psi), renormalised to norm 1 each step.
+ * Works on level 0; level 1 becomes invalid (call gpe_start again).  Replaces itp +
+ * normalize_h (Spin1GpeBCC.cu:554-662,955-967; GPUGrossPitaevskii_BCC.cu:287-325,431-443,845-849). */
+int gpe_itp(gpe_ctx *ctx, int nsteps, double dt_itp);
+
+/* FP64 observables of a level.  Replaces density/innerProduct/localAvgSpinAndDensity and
+ * the integrate trees (Spin1GpeBCC.cu:191-271,328-392,955-1020). */
+int gpe_observables(gpe_ctx *ctx, int level, gpe_observables_t *out);
+
+/* Scale a level by a real factor (normalize, Spin1GpeBCC.cu:412-434). */
+int gpe_scale(gpe_ctx *ctx, int level, double factor);
+
+/* Wait for all work of the context. */
+int gpe_sync(gpe_ctx *ctx);
+
+/* Simulation time of level 0 and the number of sweeps taken since set_state. */
+int gpe_time(const gpe_ctx *ctx, double *t, long long *sweeps);
+
+/* Average duration in ms of the step kernel over the last gpe_step call, measured with
+ * HIP events on the context's stream; bytes_per_sweep gets the algorithmic bytes of one
+ * sweep (3 touches x ncomp x 2 x sizeof(real) per unknown). */
+int gpe_last_step_timing(const gpe_ctx *ctx, double *ms_per_sweep, double *bytes_per_sweep);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,652 @@
+// Host runtime of libgpe: contexts, device memory, lattice tables, field ramps and the
+// sweep loop behind the C ABI declared in include/gpe.h.
+#include "gpe.h"
+#include "gpe_internal.h"
+#include "lattice_tables.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+using namespace gpe;
+
+namespace {
+
+struct Ramp {
+    std::vector<double> q_val, q_dur, b_val, b_dur;
+    std::vector<int> q_type, b_type;
+    double tau = 1.0, time_scale = 1.0;
+    double scale_q = 1.0, scale_b = 1.0, qscale_q = 0.0, qscale_b = 0.0;
+    bool active = false;
+};
+
+// One channel of the piecewise ramp (getSignal, Spin1GpeBCC/AliceRingRamps.h:80-138).
+void ramp_channel(const std::vector<double> &val, const std::vector<double> &dur, const std::vector<int> &type,
+                  int width, double tau, double t, double *out)
+{
+    const int n = (int)dur.size();
+    if (n == 0) {
+        for (int k = 0; k < width; ++k) out[k] = 0.0;
+        return;
+    }
+    int idx = 0;
+    for (; idx < n; ++idx) {
+        const double tin = t - dur[idx];
+        if (tin < 0) break;
+        t = tin;
+    }
+    if (idx >= n) {
+        for (int k = 0; k < width; ++k) out[k] = val[(n - 1) * width + k];
+        return;
+    }
+    for (int k = 0; k < width; ++k) {
+        const double prev = idx > 0 ? val[(idx - 1) * width + k] : 0.0;
+        const double cur = val[idx * width + k];
+        switch (type[idx]) {
+        case 1: out[k] = prev + t * (cur - prev) / dur[idx]; break;
+        case 2: out[k] = prev + (cur - prev) * std::sqrt(t / dur[idx]); break;
+        case 3: out[k] = prev + (cur - prev) * (1.0 - std::exp(-t / tau)); break;
+        default: out[k] = cur; break;
+        }
+    }
+}
+
+}  // namespace
+
+struct gpe_ctx {
+    gpe_config cfg{};
+    const LatticeTable *lat = nullptr;
+    Geometry g{};
+    Physics ph{};
+    int fp64 = 0;
+    size_t elem_bytes = 0;   // bytes of one complex element
+    long long level_elems = 0;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    void *level[2] = {nullptr, nullptr};
+    // tables
+    int *d_face = nullptr;
+    float *d_w32 = nullptr, *d_wsum32 = nullptr, *d_pos32 = nullptr;
+    double *d_w64 = nullptr, *d_wsum64 = nullptr, *d_pos64 = nullptr, *d_dv = nullptr;
+    // fields per sweep
+    FieldRec *d_fields = nullptr;
+    int fields_cap = 0;
+    FieldRec *h_fields = nullptr;  // pinned staging
+    // reductions
+    double *d_partials = nullptr;
+    double *d_scalars = nullptr;   // [0..15] results / scales
+    // static field and ramp
+    double bq = 0, bb[3] = {0, 0, 0}, bqq = 0, bbq[3] = {0, 0, 0};
+    Ramp ramp;
+    // time bookkeeping
+    double t0 = 0.0;
+    long long sweeps = 0;
+    bool level1_valid = false;
+    // timing
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms_per_sweep = 0.0;
+    std::string err;
+};
+
+namespace {
+
+int fail(gpe_ctx *c, int code, const char *what)
+{
+    if (c) c->err = what;
+    return code;
+}
+
+int hip_fail(gpe_ctx *c, hipError_t e, const char *where)
+{
+    if (c) {
+        c->err = std::string(where) + ": " + hipGetErrorString(e);
+    }
+    return GPE_EDEVICE;
+}
+
+#define HIPCALL(ctx, call)                                     \
+    do {                                                       \
+        hipError_t e_ = (call);                                \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, #call); \
+    } while (0)
+
+double sweep_dt(const gpe_ctx *c) { return c->cfg.scheme == GPE_SCHEME_STAGGERED ? 0.5 * c->cfg.dt : c->cfg.dt; }
+
+FieldRec field_at(const gpe_ctx *c, double t)
+{
+    FieldRec f{};
+    f.t = t;
+    if (c->ramp.active) {
+        const Ramp &r = c->ramp;
+        double sq, sb[3];
+        const double tr = t * r.time_scale;
+        ramp_channel(r.q_val, r.q_dur, r.q_type, 1, r.tau, tr, &sq);
+        ramp_channel(r.b_val, r.b_dur, r.b_type, 3, r.tau, tr, sb);
+        f.bq = r.scale_q * sq;
+        f.bqq = r.qscale_q * sq;
+        for (int k = 0; k < 3; ++k) {
+            f.bb[k] = r.scale_b * sb[k];
+            f.bbq[k] = r.qscale_b * sb[k];
+        }
+    } else {
+        f.bq = c->bq;
+        f.bqq = c->bqq;
+        for (int k = 0; k < 3; ++k) {
+            f.bb[k] = c->bb[k];
+            f.bbq[k] = c->bbq[k];
+        }
+    }
+    return f;
+}
+
+bool quad_active(const gpe_ctx *c)
+{
+    if (c->cfg.ncomp == 1) return false;
+    if (c->ramp.active) return c->ramp.qscale_q != 0.0 || c->ramp.qscale_b != 0.0;
+    return c->bqq != 0.0 || c->bbq[0] != 0.0 || c->bbq[1] != 0.0 || c->bbq[2] != 0.0;
+}
+
+Tables tables(const gpe_ctx *c)
+{
+    Tables t;
+    t.face = c->d_face;
+    t.w32 = c->d_w32;
+    t.w64 = c->d_w64;
+    t.wsum32 = c->d_wsum32;
+    t.wsum64 = c->d_wsum64;
+    t.pos32 = c->d_pos32;
+    t.pos64 = c->d_pos64;
+    t.dv = c->d_dv;
+    return t;
+}
+
+int ensure_fields(gpe_ctx *c, int n)
+{
+    if (n <= c->fields_cap) return GPE_OK;
+    int cap = c->fields_cap ? c->fields_cap : 1024;
+    while (cap < n) cap *= 2;
+    if (c->d_fields) {
+        HIPCALL(c, hipStreamSynchronize(c->stream));
+        (void)hipFree(c->d_fields);
+        (void)hipHostFree(c->h_fields);
+        c->d_fields = nullptr;
+        c->h_fields = nullptr;
+    }
+    HIPCALL(c, hipMalloc((void **)&c->d_fields, sizeof(FieldRec) * cap));
+    HIPCALL(c, hipHostMalloc((void **)&c->h_fields, sizeof(FieldRec) * cap, hipHostMallocDefault));
+    c->fields_cap = cap;
+    return GPE_OK;
+}
+
+// Upload the fields of sweeps [first, first + n) to d_fields[0..n).
+int upload_fields(gpe_ctx *c, long long first, int n)
+{
+    int rc = ensure_fields(c, n);
+    if (rc) return rc;
+    HIPCALL(c, hipStreamSynchronize(c->stream));  // staging buffer may still feed a copy
+    const double sdt = sweep_dt(c);
+    for (int k = 0; k < n; ++k) c->h_fields[k] = field_at(c, c->t0 + (double)(first + k) * sdt);
+    HIPCALL(c, hipMemcpyAsync(c->d_fields, c->h_fields, sizeof(FieldRec) * n, hipMemcpyHostToDevice, c->stream));
+    return GPE_OK;
+}
+
+SweepArgs sweep_args(gpe_ctx *c, int mode_coef_sweep, const void *cur, void *next, double coef)
+{
+    SweepArgs a;
+    a.g = c->g;
+    a.tb = tables(c);
+    a.ph = c->ph;
+    a.cur = cur;
+    a.next = next;
+    a.fields = c->d_fields;
+    a.sweep = mode_coef_sweep;
+    a.quad = quad_active(c) ? 1 : 0;
+    a.loss = c->cfg.alpha != 0.0 ? 1 : 0;
+    a.coef = coef;
+    a.scale_in = nullptr;
+    a.partials = c->d_partials;
+    return a;
+}
+
+int check_level(gpe_ctx *c, int level)
+{
+    if (level != 0 && level != 1) return fail(c, GPE_EINVAL, "level must be 0 or 1");
+    return GPE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gpe_lattice_values(int lattice)
+{
+    if (lattice < 0 || lattice > 4) return GPE_EINVAL;
+    return kLattices[lattice].V;
+}
+
+const char *gpe_last_error(const gpe_ctx *ctx)
+{
+    if (!ctx) return "null context";
+    return ctx->err.c_str();
+}
+
+int gpe_create(const gpe_config *cfg, gpe_ctx **out)
+{
+    if (!cfg || !out) return GPE_EINVAL;
+    *out = nullptr;
+    if (cfg->lattice < 0 || cfg->lattice > 4) return GPE_EINVAL;
+    if (cfg->ncomp != 1 && cfg->ncomp != 3 && cfg->ncomp != 5) return GPE_EINVAL;
+    if (cfg->precision != GPE_FP32 && cfg->precision != GPE_FP64) return GPE_EINVAL;
+    if (cfg->scheme != GPE_SCHEME_LEAPFROG3 && cfg->scheme != GPE_SCHEME_STAGGERED) return GPE_EINVAL;
+    if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1) return GPE_EINVAL;
+    if (!(cfg->block_scale > 0.0) || !(cfg->dt > 0.0)) return GPE_EINVAL;
+    gpe_ctx *c = new (std::nothrow) gpe_ctx();
+    if (!c) return GPE_ENOMEM;
+    c->cfg = *cfg;
+    c->lat = &kLattices[cfg->lattice];
+    c->fp64 = cfg->precision == GPE_FP64;
+    c->elem_bytes = c->fp64 ? 16 : 8;
+
+    Geometry &g = c->g;
+    g.nx = cfg->nx;
+    g.ny = cfg->ny;
+    g.nz = cfg->nz;
+    g.V = c->lat->V;
+    g.deg = c->lat->deg;
+    g.nseg = (cfg->nx + 63) / 64;
+    g.ty = cfg->ny < 8 ? cfg->ny : 8;
+    g.ntile = (cfg->ny + g.ty - 1) / g.ty;
+    const long long align = c->fp64 ? 8 : 16;  // 128-byte line starts
+    g.lx = ((long long)cfg->nx + align - 1) / align * align;
+    g.waves = (long long)g.ntile * g.ty * g.nz * g.V * g.nseg;
+    const long long blocks = (g.waves + 3) / 4;
+    if (blocks > 0x7fffffffLL) {
+        delete c;
+        return GPE_EINVAL;
+    }
+    g.blocks = (int)blocks;
+    c->level_elems = (long long)cfg->ncomp * g.nz * g.ny * g.V * g.lx;
+
+    Physics &ph = c->ph;
+    ph.cell = cfg->block_scale * c->lat->block_width;
+    for (int k = 0; k < 3; ++k) {
+        ph.p0[k] = cfg->p0[k];
+        ph.trap2[k] = cfg->trap[k] * cfg->trap[k];
+        ph.center[k] = cfg->center[k];
+    }
+    ph.trap_offset = cfg->trap_offset;
+    ph.trap_off_time = cfg->trap_off_time;
+    ph.c0 = cfg->c0;
+    ph.c1 = cfg->c1;
+    ph.c2 = cfg->c2;
+    ph.alpha = cfg->alpha;
+    ph.quad_sign = cfg->quad_sign;
+    ph.q = cfg->q;
+
+    int rc = GPE_OK;
+    hipError_t e;
+    if ((e = hipGetDevice(&c->device)) != hipSuccess) rc = hip_fail(c, e, "hipGetDevice");
+    if (!rc && (e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
+        rc = hip_fail(c, e, "hipStreamCreate");
+    const size_t lbytes = (size_t)c->level_elems * c->elem_bytes;
+    for (int l = 0; l < 2 && !rc; ++l) {
+        if ((e = hipMalloc(&c->level[l], lbytes)) != hipSuccess)
+            rc = GPE_ENOMEM;
+        else if ((e = hipMemsetAsync(c->level[l], 0, lbytes, c->stream)) != hipSuccess)
+            rc = hip_fail(c, e, "hipMemset");
+    }
+    // tables: weights w_f = -1/2 hodge_f / s^2 (Spin1GpeBCC.cu:859,1169)
+    const int V = g.V, deg = g.deg;
+    const double s = cfg->block_scale;
+    std::vector<double> w64(V * deg), wsum64(V, 0.0), pos64(3 * V), dv(V);
+    std::vector<float> w32(V * deg), wsum32(V), pos32(3 * V);
+    for (int v = 0; v < V; ++v) {
+        for (int f = 0; f < deg; ++f) {
+            const double w = -0.5 * c->lat->hodge[v * deg + f] / (s * s);
+            w64[v * deg + f] = w;
+            w32[v * deg + f] = (float)w;
+            wsum64[v] += w;
+        }
+        wsum32[v] = (float)wsum64[v];
+        for (int k = 0; k < 3; ++k) {
+            pos64[3 * v + k] = s * c->lat->pos[3 * v + k];
+            pos32[3 * v + k] = (float)pos64[3 * v + k];
+        }
+        dv[v] = s * s * s * c->lat->vol[v];
+    }
+    auto up = [&](void **dst, const void *src, size_t bytes) {
+        if (rc) return;
+        hipError_t ee = hipMalloc(dst, bytes);
+        if (ee != hipSuccess) {
+            rc = GPE_ENOMEM;
+            return;
+        }
+        ee = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+        if (ee != hipSuccess) rc = hip_fail(c, ee, "hipMemcpy tables");
+    };
+    up((void **)&c->d_face, c->lat->face, sizeof(int) * V * deg);
+    up((void **)&c->d_w32, w32.data(), sizeof(float) * V * deg);
+    up((void **)&c->d_w64, w64.data(), sizeof(double) * V * deg);
+    up((void **)&c->d_wsum32, wsum32.data(), sizeof(float) * V);
+    up((void **)&c->d_wsum64, wsum64.data(), sizeof(double) * V);
+    up((void **)&c->d_pos32, pos32.data(), sizeof(float) * 3 * V);
+    up((void **)&c->d_pos64, pos64.data(), sizeof(double) * 3 * V);
+    up((void **)&c->d_dv, dv.data(), sizeof(double) * V);
+    if (!rc && hipMalloc((void **)&c->d_partials, sizeof(double) * OBS_COUNT * (size_t)g.blocks) != hipSuccess)
+        rc = GPE_ENOMEM;
+    if (!rc && hipMalloc((void **)&c->d_scalars, sizeof(double) * 64) != hipSuccess) rc = GPE_ENOMEM;
+    if (!rc) rc = ensure_fields(c, 1024);
+    if (!rc && (e = hipEventCreate(&c->ev0)) != hipSuccess) rc = hip_fail(c, e, "hipEventCreate");
+    if (!rc && (e = hipEventCreate(&c->ev1)) != hipSuccess) rc = hip_fail(c, e, "hipEventCreate");
+    if (!rc && (e = hipStreamSynchronize(c->stream)) != hipSuccess) rc = hip_fail(c, e, "create sync");
+    if (rc) {
+        gpe_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return GPE_OK;
+}
+
+int gpe_destroy(gpe_ctx *c)
+{
+    if (!c) return GPE_EINVAL;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int l = 0; l < 2; ++l)
+        if (c->level[l]) (void)hipFree(c->level[l]);
+    void *ptrs[] = {c->d_face, c->d_w32, c->d_w64, c->d_wsum32, c->d_wsum64, c->d_pos32,
+                    c->d_pos64, c->d_dv, c->d_fields, c->d_partials, c->d_scalars};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c->h_fields) (void)hipHostFree(c->h_fields);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return GPE_OK;
+}
+
+static int set_state_impl(gpe_ctx *c, int level, const double *psi, double t, hipMemcpyKind kind)
+{
+    if (!c) return GPE_EINVAL;
+    if (!psi) return fail(c, GPE_EINVAL, "null state pointer");
+    int rc = check_level(c, level);
+    if (rc) return rc;
+    const Geometry &g = c->g;
+    const size_t nbytes = (size_t)c->cfg.ncomp * g.nz * g.ny * g.nx * g.V * 2 * sizeof(double);
+    const double *src = psi;
+    double *staging = nullptr;
+    if (kind == hipMemcpyHostToDevice) {
+        HIPCALL(c, hipMallocAsync((void **)&staging, nbytes, c->stream));
+        HIPCALL(c, hipMemcpyAsync(staging, psi, nbytes, hipMemcpyHostToDevice, c->stream));
+        src = staging;
+    }
+    hipError_t e = launch_pack(c->fp64, g, c->cfg.ncomp, src, c->level[level], c->stream);
+    if (staging) (void)hipFreeAsync(staging, c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "pack");
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+    if (level == 0) {
+        c->t0 = t;
+        c->sweeps = 0;
+        c->level1_valid = false;
+    } else {
+        c->level1_valid = true;
+    }
+    return GPE_OK;
+}
+
+int gpe_set_state(gpe_ctx *c, int level, const double *psi, double t)
+{
+    return set_state_impl(c, level, psi, t, hipMemcpyHostToDevice);
+}
+
+int gpe_set_state_device(gpe_ctx *c, int level, const double *dpsi, double t)
+{
+    return set_state_impl(c, level, dpsi, t, hipMemcpyDeviceToDevice);
+}
+
+static int get_state_impl(gpe_ctx *c, int level, double *psi, bool host)
+{
+    if (!c) return GPE_EINVAL;
+    if (!psi) return fail(c, GPE_EINVAL, "null state pointer");
+    int rc = check_level(c, level);
+    if (rc) return rc;
+    const Geometry &g = c->g;
+    const size_t nbytes = (size_t)c->cfg.ncomp * g.nz * g.ny * g.nx * g.V * 2 * sizeof(double);
+    double *dst = psi;
+    double *staging = nullptr;
+    if (host) {
+        HIPCALL(c, hipMallocAsync((void **)&staging, nbytes, c->stream));
+        dst = staging;
+    }
+    hipError_t e = launch_unpack(c->fp64, g, c->cfg.ncomp, c->level[level], dst, c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "unpack");
+    if (host) {
+        HIPCALL(c, hipMemcpyAsync(psi, staging, nbytes, hipMemcpyDeviceToHost, c->stream));
+        (void)hipFreeAsync(staging, c->stream);
+    }
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+    return GPE_OK;
+}
+
+int gpe_get_state(gpe_ctx *c, int level, double *psi) { return get_state_impl(c, level, psi, true); }
+int gpe_get_state_device(gpe_ctx *c, int level, double *dpsi) { return get_state_impl(c, level, dpsi, false); }
+
+int gpe_set_field(gpe_ctx *c, double bq, const double bb[3], double bq_quad, const double bb_quad[3])
+{
+    if (!c) return GPE_EINVAL;
+    c->ramp.active = false;
+    c->bq = bq;
+    c->bqq = bq_quad;
+    for (int k = 0; k < 3; ++k) {
+        c->bb[k] = bb ? bb[k] : 0.0;
+        c->bbq[k] = bb_quad ? bb_quad[k] : 0.0;
+    }
+    return GPE_OK;
+}
+
+int gpe_set_ramp(gpe_ctx *c, const gpe_ramp *r)
+{
+    if (!c) return GPE_EINVAL;
+    if (!r) {
+        c->ramp.active = false;
+        return GPE_OK;
+    }
+    if (r->nq < 0 || r->nb < 0) return fail(c, GPE_EINVAL, "negative segment count");
+    if ((r->nq && (!r->q_val || !r->q_dur || !r->q_type)) || (r->nb && (!r->b_val || !r->b_dur || !r->b_type)))
+        return fail(c, GPE_EINVAL, "null ramp array");
+    Ramp &d = c->ramp;
+    d.q_val.assign(r->q_val, r->q_val + r->nq);
+    d.q_dur.assign(r->q_dur, r->q_dur + r->nq);
+    d.q_type.assign(r->q_type, r->q_type + r->nq);
+    d.b_val.assign(r->b_val, r->b_val + 3 * r->nb);
+    d.b_dur.assign(r->b_dur, r->b_dur + r->nb);
+    d.b_type.assign(r->b_type, r->b_type + r->nb);
+    for (int t : d.q_type)
+        if (t < 0 || t > 3) return fail(c, GPE_EINVAL, "bad ramp segment type");
+    for (int t : d.b_type)
+        if (t < 0 || t > 3) return fail(c, GPE_EINVAL, "bad ramp segment type");
+    d.tau = r->tau;
+    d.time_scale = r->time_scale;
+    d.scale_q = r->scale_q;
+    d.scale_b = r->scale_b;
+    d.qscale_q = r->qscale_q;
+    d.qscale_b = r->qscale_b;
+    d.active = true;
+    return GPE_OK;
+}
+
+int gpe_start(gpe_ctx *c, int mode, double mu)
+{
+    if (!c) return GPE_EINVAL;
+    const double sdt = sweep_dt(c);
+    if (mode == GPE_START_EULER) {
+        int rc = upload_fields(c, c->sweeps, 1);
+        if (rc) return rc;
+        SweepArgs a = sweep_args(c, 0, c->level[0], c->level[1], sdt);
+        hipError_t e = launch_sweep(c->fp64, c->cfg.ncomp, MODE_EULER, a, c->stream);
+        if (e != hipSuccess) return hip_fail(c, e, "forward Euler");
+        std::swap(c->level[0], c->level[1]);
+        c->sweeps += 1;
+    } else if (mode == GPE_START_PHASE) {
+        // psi(t - sdt) = exp(+i mu sdt) psi(t)  (GPBlocks.cpp:516-523)
+        hipError_t e = launch_phase_copy(c->fp64, c->level[0], c->level[1], c->level_elems, std::cos(mu * sdt),
+                                         std::sin(mu * sdt), c->stream);
+        if (e != hipSuccess) return hip_fail(c, e, "phase start");
+    } else if (mode == GPE_START_COPY) {
+        HIPCALL(c, hipMemcpyAsync(c->level[1], c->level[0], (size_t)c->level_elems * c->elem_bytes,
+                                  hipMemcpyDeviceToDevice, c->stream));
+    } else {
+        return fail(c, GPE_EINVAL, "unknown start mode");
+    }
+    c->level1_valid = true;
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+    return GPE_OK;
+}
+
+int gpe_step(gpe_ctx *c, int nsteps)
+{
+    if (!c) return GPE_EINVAL;
+    if (nsteps < 0) return fail(c, GPE_EINVAL, "negative step count");
+    if (!c->level1_valid) return fail(c, GPE_ESTATE, "second time level not set: call gpe_start");
+    if (nsteps == 0) return GPE_OK;
+    const int per = c->cfg.scheme == GPE_SCHEME_STAGGERED ? 2 : 1;
+    const long long total = (long long)nsteps * per;
+    const double coef = 2.0 * sweep_dt(c);
+    const int chunk = 1 << 16;
+    double ms_total = 0.0;
+    for (long long done = 0; done < total;) {
+        const int n = (int)std::min<long long>(chunk, total - done);
+        int rc = upload_fields(c, c->sweeps, n);
+        if (rc) return rc;
+        HIPCALL(c, hipEventRecord(c->ev0, c->stream));
+        for (int k = 0; k < n; ++k) {
+            SweepArgs a = sweep_args(c, k, c->level[0], c->level[1], coef);
+            hipError_t e = launch_sweep(c->fp64, c->cfg.ncomp, MODE_UPDATE, a, c->stream);
+            if (e != hipSuccess) return hip_fail(c, e, "sweep launch");
+            std::swap(c->level[0], c->level[1]);
+            c->sweeps += 1;
+        }
+        HIPCALL(c, hipEventRecord(c->ev1, c->stream));
+        HIPCALL(c, hipEventSynchronize(c->ev1));
+        float ms = 0.f;
+        HIPCALL(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        ms_total += ms;
+        done += n;
+    }
+    c->last_ms_per_sweep = ms_total / (double)total;
+    return GPE_OK;
+}
+
+int gpe_itp(gpe_ctx *c, int nsteps, double dt_itp)
+{
+    if (!c) return GPE_EINVAL;
+    if (nsteps < 0 || !(dt_itp > 0.0)) return fail(c, GPE_EINVAL, "bad ITP arguments");
+    int rc = upload_fields(c, c->sweeps, 1);  // field and time of level 0
+    if (rc) return rc;
+    double *norm = c->d_scalars + 0;
+    double *scale[2] = {c->d_scalars + 1, c->d_scalars + 2};
+    // initial scale from the norm of level 0
+    ObsArgs oa;
+    oa.g = c->g;
+    oa.tb = tables(c);
+    oa.ph = c->ph;
+    oa.psi = c->level[0];
+    oa.field = field_at(c, c->t0 + c->sweeps * sweep_dt(c));
+    oa.quad = quad_active(c);
+    oa.partials = c->d_partials;
+    hipError_t e = launch_observables(c->fp64, c->cfg.ncomp, oa, c->stream);
+    if (e == hipSuccess) e = launch_finalize(c->d_partials, c->g.blocks, OBS_COUNT, OBS_MAXDENS, c->d_scalars + 8, c->stream);
+    if (e == hipSuccess) e = launch_norm_to_scale(c->d_scalars + 8 + OBS_NORM, 1.0, scale[0], c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "ITP initial norm");
+    for (int i = 0; i < nsteps; ++i) {
+        SweepArgs a = sweep_args(c, 0, c->level[0], c->level[1], dt_itp);
+        a.scale_in = scale[i & 1];
+        e = launch_sweep(c->fp64, c->cfg.ncomp, MODE_ITP, a, c->stream);
+        if (e == hipSuccess) e = launch_finalize(c->d_partials, c->g.blocks, 1, -1, norm, c->stream);
+        if (e == hipSuccess) e = launch_norm_to_scale(norm, 1.0, scale[(i + 1) & 1], c->stream);
+        if (e != hipSuccess) return hip_fail(c, e, "ITP step");
+        std::swap(c->level[0], c->level[1]);
+    }
+    e = launch_scale(c->fp64, c->level[0], c->level_elems, 1.0, scale[nsteps & 1], c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "ITP final scale");
+    c->level1_valid = false;
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+    return GPE_OK;
+}
+
+int gpe_observables(gpe_ctx *c, int level, gpe_observables_t *out)
+{
+    if (!c) return GPE_EINVAL;
+    if (!out) return fail(c, GPE_EINVAL, "null output");
+    int rc = check_level(c, level);
+    if (rc) return rc;
+    const double t = c->t0 + (double)(c->sweeps - level) * sweep_dt(c);
+    ObsArgs oa;
+    oa.g = c->g;
+    oa.tb = tables(c);
+    oa.ph = c->ph;
+    oa.psi = c->level[level];
+    oa.field = field_at(c, t);
+    oa.quad = quad_active(c);
+    oa.partials = c->d_partials;
+    hipError_t e = launch_observables(c->fp64, c->cfg.ncomp, oa, c->stream);
+    if (e == hipSuccess) e = launch_finalize(c->d_partials, c->g.blocks, OBS_COUNT, OBS_MAXDENS, c->d_scalars + 8, c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "observables");
+    double h[OBS_COUNT];
+    HIPCALL(c, hipMemcpyAsync(h, c->d_scalars + 8, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+    out->norm = h[OBS_NORM];
+    out->energy = h[OBS_ENERGY];
+    out->expect_h = h[OBS_EXPECT_H];
+    out->magnetization[0] = h[OBS_MX];
+    out->magnetization[1] = h[OBS_MY];
+    out->magnetization[2] = h[OBS_MZ];
+    out->max_density = h[OBS_MAXDENS];
+    out->time = t;
+    out->finite = (h[OBS_NONFINITE] == 0.0 && std::isfinite(h[OBS_NORM]) && std::isfinite(h[OBS_ENERGY])) ? 1 : 0;
+    return out->finite ? GPE_OK : fail(c, GPE_ERANGE, "non-finite value in the state");
+}
+
+int gpe_scale(gpe_ctx *c, int level, double factor)
+{
+    if (!c) return GPE_EINVAL;
+    int rc = check_level(c, level);
+    if (rc) return rc;
+    hipError_t e = launch_scale(c->fp64, c->level[level], c->level_elems, factor, nullptr, c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "scale");
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+    return GPE_OK;
+}
+
+int gpe_sync(gpe_ctx *c)
+{
+    if (!c) return GPE_EINVAL;
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+    return GPE_OK;
+}
+
+int gpe_time(const gpe_ctx *c, double *t, long long *sweeps)
+{
+    if (!c) return GPE_EINVAL;
+    if (t) *t = c->t0 + (double)c->sweeps * sweep_dt(c);
+    if (sweeps) *sweeps = c->sweeps;
+    return GPE_OK;
+}
+
+int gpe_last_step_timing(const gpe_ctx *c, double *ms_per_sweep, double *bytes_per_sweep)
+{
+    if (!c) return GPE_EINVAL;
+    if (ms_per_sweep) *ms_per_sweep = c->last_ms_per_sweep;
+    if (bytes_per_sweep) {
+        const double n = (double)c->cfg.nx * c->cfg.ny * c->cfg.nz * c->g.V;
+        *bytes_per_sweep = n * 3.0 * c->cfg.ncomp * 2.0 * (c->fp64 ? 8.0 : 4.0);
+    }
+    return GPE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,103 @@
+// Internal interfaces between the host runtime (gpe_host.cpp) and the HIP kernels
+// (gpe_kernels.hip).  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gpe {
+
+// Device layout of one time level: component-major, then z, y, unknown v, and x fastest,
+//   element(c, z, y, v, x) = (((c * nz + z) * ny + y) * V + v) * lx + x
+// with lx >= nx rounded up so every x-line starts on a 128-byte boundary.  One wave
+// handles 64 consecutive x of one (z, y, v) line, so the unknown v, the face table row and
+// the neighbour rows are uniform across the wave and every neighbour load is a contiguous
+// 64-lane access.  Out-of-box neighbours read as zero (Dirichlet, as the reference's
+// never-written halo).
+struct Geometry {
+    int nx, ny, nz, V, deg;
+    int nseg;          // ceil(nx / 64) wave segments per x-line
+    int ty;            // y rows per tile of the traversal order
+    int ntile;         // ceil(ny / ty)
+    long long lx;      // elements per x-line
+    long long waves;   // total waves of a sweep
+    int blocks;        // workgroups of a sweep (4 waves each)
+};
+
+// Per-context constant tables on the device (weights already scaled by -1/2 / s^2).
+struct Tables {
+    const int *face;       // V*deg packed faces
+    const float *w32;      // V*deg
+    const double *w64;     // V*deg
+    const float *wsum32;   // V: sum of the weights of unknown v
+    const double *wsum64;  // V
+    const float *pos32;    // V*3: s * position of unknown v inside its block
+    const double *pos64;   // V*3
+    const double *dv;      // V: s^3 * dual volume of unknown v
+};
+
+struct Physics {
+    double cell;           // s * block_width
+    double p0[3];
+    double trap2[3];       // squared trap frequencies
+    double trap_offset, trap_off_time;
+    double c0, c1, c2, alpha, quad_sign, q;
+    double center[3];
+};
+
+// Field and time of one sweep (computed on the host from the ramp).
+struct FieldRec {
+    double bq, bb[3];
+    double bqq, bbq[3];
+    double t;
+    double pad;
+};
+
+enum Mode { MODE_UPDATE = 0, MODE_EULER = 1, MODE_ITP = 2, MODE_HPSI = 3 };
+
+struct SweepArgs {
+    Geometry g;
+    Tables tb;
+    Physics ph;
+    const void *cur;          // level read by the stencil (psi_k)
+    void *next;               // level written (psi_{k-1} -> psi_{k+1})
+    const FieldRec *fields;   // field of sweep k at fields[k]
+    int sweep;                // index into fields
+    int quad, loss;           // quadratic Zeeman / three-body loss active
+    double coef;              // update: next += coef * (-i) H cur ; ITP: dt
+    const double *scale_in;   // ITP: scale applied to cur before H (device scalar)
+    double *partials;         // ITP: one FP64 partial norm per workgroup
+};
+
+// Observables partial sums per workgroup.
+enum { OBS_NORM = 0, OBS_ENERGY, OBS_EXPECT_H, OBS_MX, OBS_MY, OBS_MZ, OBS_MAXDENS, OBS_NONFINITE, OBS_COUNT };
+
+struct ObsArgs {
+    Geometry g;
+    Tables tb;
+    Physics ph;
+    const void *psi;
+    FieldRec field;
+    int quad;
+    double *partials;         // OBS_COUNT per workgroup
+};
+
+// launchers (gpe_kernels.hip)
+hipError_t launch_sweep(int fp64, int ncomp, int mode, const SweepArgs &a, hipStream_t s);
+hipError_t launch_observables(int fp64, int ncomp, const ObsArgs &a, hipStream_t s);
+// sum `n` partial records of `width` doubles into out[width] (max for column maxcol, if >= 0)
+hipError_t launch_finalize(const double *partials, int n, int width, int maxcol, double *out, hipStream_t s);
+// out = sqrt(target / in[0])
+hipError_t launch_norm_to_scale(const double *in, double target, double *out, hipStream_t s);
+// level *= factor (device scalar if dfactor != nullptr)
+hipError_t launch_scale(int fp64, void *psi, long long nelem, double factor, const double *dfactor,
+                        hipStream_t s);
+// ABI complex128 [c][z][y][x][v] <-> device layout
+hipError_t launch_pack(int fp64, const Geometry &g, int ncomp, const double *abi, void *dev, hipStream_t s);
+hipError_t launch_unpack(int fp64, const Geometry &g, int ncomp, const void *dev, double *abi, hipStream_t s);
+// dst = phase * src (complex), for the staggered start
+hipError_t launch_phase_copy(int fp64, const void *src, void *dst, long long nelem, double pre, double pim,
+                             hipStream_t s);
+
+}  // namespace gpe

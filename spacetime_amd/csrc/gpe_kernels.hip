@@ -1,0 +1,728 @@
+// HIP kernels of the DEC Gross-Pitaevskii step for CDNA4 (gfx950).
+//
+// One sweep = one pass over every unknown of a time level:
+//     next += coef * (-i) H(cur)           (3-level leapfrog, Spin1GpeBCC.cu:793-917;
+//                                           staggered half-step, GPBlocks.cpp:560-606)
+// with H = -1/2 L + V(r) + c0 n [-i alpha n^2] + (c1 F +- B).f + singlet + quadratic Zeeman.
+//
+// Work mapping: a wave owns 64 consecutive x of one (z, y, unknown v) line of the device
+// layout (gpe_internal.h), so the face table row, the neighbour line (z+dz, y+dy, target)
+// and the field are wave-uniform (scalar loads), and each neighbour access is one
+// contiguous 64-lane load shifted by dx.  Out-of-box neighbours are folded into the face
+// weight (weight 0 and a clamped in-bounds address), so the sweep has no branches.
+// Workgroups are remapped so that each XCD walks a contiguous slab of the traversal
+// order (y-tiles, then z, then rows of the tile): the z +- 1 and y +- 1 rows a wave needs
+// were fetched into the same XCD's L2 by its neighbours shortly before.
+#include "gpe_internal.h"
+
+#include <type_traits>
+
+namespace gpe {
+
+template <typename R>
+using vec2_t = typename std::conditional<std::is_same<R, float>::value, float2, double2>::type;
+
+__device__ __forceinline__ int logical_block(int b, int nb)
+{
+    // hardware dispatch is round-robin over the 8 XCDs: give XCD k a contiguous range
+    const int q = nb >> 3, r = nb & 7, k = b & 7;
+    return k * q + (k < r ? k : r) + (b >> 3);
+}
+
+struct WaveCoord {
+    int x, v, y, z;
+    bool valid;
+};
+
+__device__ __forceinline__ WaveCoord decode(const Geometry &g)
+{
+    WaveCoord wc;
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const long long w = (long long)logical_block(blockIdx.x, gridDim.x) * 4 + wid;
+    long long r = w;
+    const int xseg = (int)(r % g.nseg);
+    r /= g.nseg;
+    wc.v = (int)(r % g.V);
+    r /= g.V;
+    const int yin = (int)(r % g.ty);
+    r /= g.ty;
+    wc.z = (int)(r % g.nz);
+    const int tile = (int)(r / g.nz);
+    wc.y = tile * g.ty + yin;
+    wc.valid = (w < g.waves) && (wc.y < g.ny);
+    wc.x = xseg * 64 + (int)(threadIdx.x & 63);
+    return wc;
+}
+
+__device__ __forceinline__ long long line_index(const Geometry &g, int c, int z, int y, int v)
+{
+    return ((long long)((c * g.nz + z) * g.ny + y) * g.V + v) * g.lx;
+}
+
+// ------------------------------------------------------------------ local Hamiltonian --
+
+// (W . f) psi for spin-1 (m = +1, 0, -1): Spin1GpeBCC.cu:890-895 expanded form.
+template <typename R>
+__device__ __forceinline__ void spin1_dot(R wx, R wy, R wz, const R *pr, const R *pi, R *orr, R *oi)
+{
+    const R s = (R)0.70710678118654752440;
+    const R pre = s * wx, pim = s * wy;  // W+ / sqrt2 = (pre, pim); W- / sqrt2 = (pre, -pim)
+    // m = +1: wz psi1 + W-/sqrt2 psi0
+    orr[0] = wz * pr[0] + pre * pr[1] + pim * pi[1];
+    oi[0] = wz * pi[0] + pre * pi[1] - pim * pr[1];
+    // m = 0: W+/sqrt2 psi1 + W-/sqrt2 psi-1
+    orr[1] = pre * pr[0] - pim * pi[0] + pre * pr[2] + pim * pi[2];
+    oi[1] = pre * pi[0] + pim * pr[0] + pre * pi[2] - pim * pr[2];
+    // m = -1: W+/sqrt2 psi0 - wz psi-1
+    orr[2] = pre * pr[1] - pim * pi[1] - wz * pr[2];
+    oi[2] = pre * pi[1] + pim * pr[1] - wz * pi[2];
+}
+
+// (W . f) psi for spin-2 (m = 2..-2): ladder factors 2, sqrt6, sqrt6, 2 (Spin2GpeBCC.cu:954-970).
+template <typename R>
+__device__ __forceinline__ void spin2_dot(R wx, R wy, R wz, const R *pr, const R *pi, R *orr, R *oi)
+{
+    const R s6 = (R)2.44948974278317809820;
+    const R hr = (R)0.5 * wx, hi = (R)0.5 * wy;  // W+/2 = (hr, hi), W-/2 = (hr, -hi)
+    // lower(m) = W-/2 * a_m * psi_{m-1}; upper(m) = W+/2 * b_m * psi_{m+1}
+#define GPE_LOWER(idx, a, k) \
+    (a) * (hr * pr[k] + hi * pi[k]), (a) * (hr * pi[k] - hi * pr[k])
+#define GPE_UPPER(idx, b, k) \
+    (b) * (hr * pr[k] - hi * pi[k]), (b) * (hr * pi[k] + hi * pr[k])
+    R lr, li, ur, ui;
+    // m = 2
+    {
+        R a0 = 2 * (hr * pr[1] + hi * pi[1]), a1 = 2 * (hr * pi[1] - hi * pr[1]);
+        orr[0] = 2 * wz * pr[0] + a0;
+        oi[0] = 2 * wz * pi[0] + a1;
+    }
+    // m = 1
+    lr = s6 * (hr * pr[2] + hi * pi[2]); li = s6 * (hr * pi[2] - hi * pr[2]);
+    ur = 2 * (hr * pr[0] - hi * pi[0]); ui = 2 * (hr * pi[0] + hi * pr[0]);
+    orr[1] = wz * pr[1] + lr + ur;
+    oi[1] = wz * pi[1] + li + ui;
+    // m = 0
+    lr = s6 * (hr * pr[3] + hi * pi[3]); li = s6 * (hr * pi[3] - hi * pr[3]);
+    ur = s6 * (hr * pr[1] - hi * pi[1]); ui = s6 * (hr * pi[1] + hi * pr[1]);
+    orr[2] = lr + ur;
+    oi[2] = li + ui;
+    // m = -1
+    lr = 2 * (hr * pr[4] + hi * pi[4]); li = 2 * (hr * pi[4] - hi * pr[4]);
+    ur = s6 * (hr * pr[2] - hi * pi[2]); ui = s6 * (hr * pi[2] + hi * pr[2]);
+    orr[3] = -wz * pr[3] + lr + ur;
+    oi[3] = -wz * pi[3] + li + ui;
+    // m = -2
+    ur = 2 * (hr * pr[3] - hi * pi[3]); ui = 2 * (hr * pi[3] + hi * pr[3]);
+    orr[4] = -2 * wz * pr[4] + ur;
+    oi[4] = -2 * wz * pi[4] + ui;
+#undef GPE_LOWER
+#undef GPE_UPPER
+}
+
+template <typename R, int C>
+__device__ __forceinline__ void spin_dot(R wx, R wy, R wz, const R *pr, const R *pi, R *orr, R *oi)
+{
+    if constexpr (C == 3)
+        spin1_dot<R>(wx, wy, wz, pr, pi, orr, oi);
+    else
+        spin2_dot<R>(wx, wy, wz, pr, pi, orr, oi);
+}
+
+struct LocalCoef {
+    double vtrap;
+    double Bx, By, Bz;
+    double Qx, Qy, Qz;
+};
+
+// hr/hi += local part of H psi at one unknown.
+template <typename R, int C>
+__device__ __forceinline__ void add_local_h(const R *pr, const R *pi, R vtrap, R Bx, R By, R Bz, R Qx, R Qy, R Qz,
+                                            R c0, R c1, R c2, R alpha, R quad_sign, R q, int quad, int loss,
+                                            R *hr, R *hi)
+{
+    R n = 0;
+#pragma unroll
+    for (int c = 0; c < C; ++c) n += pr[c] * pr[c] + pi[c] * pi[c];
+    const R d = vtrap + c0 * n;
+    const R l = loss ? alpha * n * n : (R)0;
+    // (d - i l) psi
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        hr[c] += d * pr[c] + l * pi[c];
+        hi[c] += d * pi[c] - l * pr[c];
+    }
+    if constexpr (C > 1) {
+        R Fx, Fy, Fz, wx, wy, wz;
+        if constexpr (C == 3) {
+            // F+ = sqrt2 (conj(psi1) psi0 + conj(psi0) psi-1), Fz = |psi1|^2 - |psi-1|^2
+            const R s2 = (R)1.41421356237309504880;
+            Fx = s2 * (pr[0] * pr[1] + pi[0] * pi[1] + pr[1] * pr[2] + pi[1] * pi[2]);
+            Fy = s2 * (pr[0] * pi[1] - pi[0] * pr[1] + pr[1] * pi[2] - pi[1] * pr[2]);
+            Fz = pr[0] * pr[0] + pi[0] * pi[0] - pr[2] * pr[2] - pi[2] * pi[2];
+            wx = Bx + c1 * Fx;
+            wy = By + c1 * Fy;
+            wz = Bz + c1 * Fz;
+        } else {
+            const R s6 = (R)2.44948974278317809820;
+            // F+ = 2 c(psi2) psi1 + s6 c(psi1) psi0 + s6 c(psi0) psi-1 + 2 c(psi-1) psi-2
+            Fx = 2 * (pr[0] * pr[1] + pi[0] * pi[1]) + s6 * (pr[1] * pr[2] + pi[1] * pi[2]) +
+                 s6 * (pr[2] * pr[3] + pi[2] * pi[3]) + 2 * (pr[3] * pr[4] + pi[3] * pi[4]);
+            Fy = 2 * (pr[0] * pi[1] - pi[0] * pr[1]) + s6 * (pr[1] * pi[2] - pi[1] * pr[2]) +
+                 s6 * (pr[2] * pi[3] - pi[2] * pr[3]) + 2 * (pr[3] * pi[4] - pi[3] * pr[4]);
+            Fz = 2 * (pr[0] * pr[0] + pi[0] * pi[0]) + pr[1] * pr[1] + pi[1] * pi[1] - pr[3] * pr[3] -
+                 pi[3] * pi[3] - 2 * (pr[4] * pr[4] + pi[4] * pi[4]);
+            // spin-2 linear Zeeman enters with -B (Spin2GpeBCC.cu:942-957)
+            wx = c1 * Fx - Bx;
+            wy = c1 * Fy - By;
+            wz = c1 * Fz - Bz;
+        }
+        R tr[C], ti[C];
+        spin_dot<R, C>(wx, wy, wz, pr, pi, tr, ti);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            hr[c] += tr[c];
+            hi[c] += ti[c];
+        }
+        if constexpr (C == 5) {
+            // singlet: H_m += (c2/5) Theta (-1)^m conj(psi_-m), Theta = 2 p2 p-2 - 2 p1 p-1 + p0^2
+            const R thr = 2 * (pr[0] * pr[4] - pi[0] * pi[4]) - 2 * (pr[1] * pr[3] - pi[1] * pi[3]) +
+                          (pr[2] * pr[2] - pi[2] * pi[2]);
+            const R thi = 2 * (pr[0] * pi[4] + pi[0] * pr[4]) - 2 * (pr[1] * pi[3] + pi[1] * pr[3]) +
+                          2 * pr[2] * pi[2];
+            const R k = c2 * (R)0.2;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const R sg = (c & 1) ? -k : k;
+                const R cr = pr[C - 1 - c], ci = -pi[C - 1 - c];
+                hr[c] += sg * (thr * cr - thi * ci);
+                hi[c] += sg * (thr * ci + thi * cr);
+            }
+        }
+        if (quad) {
+            R ur[C], ui[C];
+            spin_dot<R, C>(Qx, Qy, Qz, pr, pi, tr, ti);
+            spin_dot<R, C>(Qx, Qy, Qz, tr, ti, ur, ui);
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                hr[c] += quad_sign * ur[c];
+                hi[c] += quad_sign * ui[c];
+            }
+        }
+        constexpr int F = (C - 1) / 2;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const R mm = (R)((F - c) * (F - c));
+            hr[c] += q * mm * pr[c];
+            hi[c] += q * mm * pi[c];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------- the sweep --
+
+template <typename R>
+__device__ __forceinline__ const R *table_w(const Tables &t);
+template <>
+__device__ __forceinline__ const float *table_w<float>(const Tables &t) { return t.w32; }
+template <>
+__device__ __forceinline__ const double *table_w<double>(const Tables &t) { return t.w64; }
+template <typename R>
+__device__ __forceinline__ const R *table_wsum(const Tables &t);
+template <>
+__device__ __forceinline__ const float *table_wsum<float>(const Tables &t) { return t.wsum32; }
+template <>
+__device__ __forceinline__ const double *table_wsum<double>(const Tables &t) { return t.wsum64; }
+template <typename R>
+__device__ __forceinline__ const R *table_pos(const Tables &t);
+template <>
+__device__ __forceinline__ const float *table_pos<float>(const Tables &t) { return t.pos32; }
+template <>
+__device__ __forceinline__ const double *table_pos<double>(const Tables &t) { return t.pos64; }
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+template <typename R, int C, int DEG, int MODE>
+__global__ __launch_bounds__(256) void k_sweep(SweepArgs a)
+{
+    using V2 = vec2_t<R>;
+    const Geometry &g = a.g;
+    const WaveCoord wc = decode(g);
+    double part = 0.0;
+
+    if (wc.valid) {
+        const int x = wc.x, v = wc.v, y = wc.y, z = wc.z;
+        const bool act = x < g.nx;
+        const int xc = act ? x : g.nx - 1;
+        const V2 *__restrict__ cur = (const V2 *)a.cur;
+        V2 *__restrict__ nxt = (V2 *)a.next;
+
+        R pr[C], pi[C], ar[C], ai[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const V2 o = cur[line_index(g, c, z, y, v) + xc];
+            pr[c] = o.x;
+            pi[c] = o.y;
+            ar[c] = 0;
+            ai[c] = 0;
+        }
+        // stencil: sum_f w_f psi_t(f), out-of-box neighbours have weight 0
+        const int *face = a.tb.face + v * DEG;
+        const R *wt = table_w<R>(a.tb) + v * DEG;
+#pragma unroll
+        for (int f = 0; f < DEG; ++f) {
+            const int pk = face[f];
+            const int dx = (pk & 3) - 1, dy = ((pk >> 2) & 3) - 1, dz = ((pk >> 4) & 3) - 1, t = pk >> 8;
+            int zn = z + dz, yn = y + dy;
+            const bool inside = (unsigned)zn < (unsigned)g.nz && (unsigned)yn < (unsigned)g.ny;
+            if (!inside) {
+                zn = z;
+                yn = y;
+            }
+            const int xn = x + dx;
+            const bool xok = (unsigned)xn < (unsigned)g.nx;
+            const int xnc = xn < 0 ? 0 : (xn >= g.nx ? g.nx - 1 : xn);
+            const R wf = (inside && xok) ? wt[f] : (R)0;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const V2 nb = cur[line_index(g, c, zn, yn, t) + xnc];
+                ar[c] += wf * nb.x;
+                ai[c] += wf * nb.y;
+            }
+        }
+        const R wsum = table_wsum<R>(a.tb)[v];
+        R hr[C], hi[C];
+        R sc = (R)1;
+        if constexpr (MODE == MODE_ITP) sc = (R)(*a.scale_in);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            hr[c] = sc * (ar[c] - wsum * pr[c]);
+            hi[c] = sc * (ai[c] - wsum * pi[c]);
+            pr[c] *= sc;
+            pi[c] *= sc;
+        }
+        // position, trap and field
+        const Physics &ph = a.ph;
+        const FieldRec &fr = a.fields[a.sweep];
+        const R *pos = table_pos<R>(a.tb) + 3 * v;
+        const R cell = (R)ph.cell;
+        const R rx = (R)ph.p0[0] + cell * (R)x + pos[0];
+        const R ry = (R)ph.p0[1] + cell * (R)y + pos[1];
+        const R rz = (R)ph.p0[2] + cell * (R)z + pos[2];
+        R vtrap = 0;
+        if (fr.t < ph.trap_off_time)
+            vtrap = (R)0.5 * ((R)ph.trap2[0] * rx * rx + (R)ph.trap2[1] * ry * ry + (R)ph.trap2[2] * rz * rz) +
+                    (R)ph.trap_offset;
+        R Bx = 0, By = 0, Bz = 0, Qx = 0, Qy = 0, Qz = 0;
+        if constexpr (C > 1) {
+            const R dxr = rx - (R)ph.center[0], dyr = ry - (R)ph.center[1], dzr = rz - (R)ph.center[2];
+            const R bq = (R)fr.bq;
+            Bx = bq * dxr + (R)fr.bb[0];
+            By = bq * dyr + (R)fr.bb[1];
+            Bz = (R)-2 * bq * dzr + (R)fr.bb[2];
+            const R bqq = (R)fr.bqq;
+            Qx = bqq * dxr + (R)fr.bbq[0];
+            Qy = bqq * dyr + (R)fr.bbq[1];
+            Qz = (R)-2 * bqq * dzr + (R)fr.bbq[2];
+        }
+        add_local_h<R, C>(pr, pi, vtrap, Bx, By, Bz, Qx, Qy, Qz, (R)ph.c0, (R)ph.c1, (R)ph.c2, (R)ph.alpha,
+                          (R)ph.quad_sign, (R)ph.q, a.quad, a.loss, hr, hi);
+
+        const R coef = (R)a.coef;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const long long idx = line_index(g, c, z, y, v) + xc;
+            V2 o;
+            if constexpr (MODE == MODE_UPDATE) {
+                o = nxt[idx];
+                o.x += coef * hi[c];
+                o.y -= coef * hr[c];
+            } else if constexpr (MODE == MODE_EULER) {
+                o.x = pr[c] + coef * hi[c];
+                o.y = pi[c] - coef * hr[c];
+            } else if constexpr (MODE == MODE_ITP) {
+                o.x = pr[c] - coef * hr[c];
+                o.y = pi[c] - coef * hi[c];
+                if (act) part += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
+            } else {
+                o.x = hr[c];
+                o.y = hi[c];
+            }
+            if (act) nxt[idx] = o;
+        }
+        if constexpr (MODE == MODE_ITP) part *= a.tb.dv[v];
+    }
+
+    if constexpr (MODE == MODE_ITP) {
+        __shared__ double red[4];
+        part = wave_sum(part);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
+        __syncthreads();
+        if (threadIdx.x == 0) a.partials[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+    }
+}
+
+// ------------------------------------------------------------------------ observables --
+
+template <typename R, int C, int DEG>
+__global__ __launch_bounds__(256) void k_observables(ObsArgs a)
+{
+    using V2 = vec2_t<R>;
+    const Geometry &g = a.g;
+    const WaveCoord wc = decode(g);
+    double acc[OBS_COUNT];
+#pragma unroll
+    for (int i = 0; i < OBS_COUNT; ++i) acc[i] = 0.0;
+
+    if (wc.valid && wc.x < g.nx) {
+        const int x = wc.x, v = wc.v, y = wc.y, z = wc.z;
+        const V2 *__restrict__ psi = (const V2 *)a.psi;
+        double pr[C], pi[C], kr[C], ki[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const V2 o = psi[line_index(g, c, z, y, v) + x];
+            pr[c] = o.x;
+            pi[c] = o.y;
+            kr[c] = 0;
+            ki[c] = 0;
+        }
+        const int *face = a.tb.face + v * DEG;
+        const double *wt = a.tb.w64 + v * DEG;
+#pragma unroll
+        for (int f = 0; f < DEG; ++f) {
+            const int pk = face[f];
+            const int dx = (pk & 3) - 1, dy = ((pk >> 2) & 3) - 1, dz = ((pk >> 4) & 3) - 1, t = pk >> 8;
+            const int zn = z + dz, yn = y + dy, xn = x + dx;
+            if ((unsigned)zn < (unsigned)g.nz && (unsigned)yn < (unsigned)g.ny && (unsigned)xn < (unsigned)g.nx) {
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    const V2 nb = psi[line_index(g, c, zn, yn, t) + xn];
+                    kr[c] += wt[f] * ((double)nb.x - pr[c]);
+                    ki[c] += wt[f] * ((double)nb.y - pi[c]);
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    kr[c] -= wt[f] * pr[c];
+                    ki[c] -= wt[f] * pi[c];
+                }
+            }
+        }
+        const Physics &ph = a.ph;
+        const double *pos = a.tb.pos64 + 3 * v;
+        const double rx = ph.p0[0] + ph.cell * x + pos[0];
+        const double ry = ph.p0[1] + ph.cell * y + pos[1];
+        const double rz = ph.p0[2] + ph.cell * z + pos[2];
+        double vtrap = 0;
+        if (a.field.t < ph.trap_off_time)
+            vtrap = 0.5 * (ph.trap2[0] * rx * rx + ph.trap2[1] * ry * ry + ph.trap2[2] * rz * rz) + ph.trap_offset;
+        double n = 0, ekin = 0;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            n += pr[c] * pr[c] + pi[c] * pi[c];
+            ekin += pr[c] * kr[c] + pi[c] * ki[c];
+        }
+        double e = ekin + vtrap * n + 0.5 * ph.c0 * n * n;
+        double eh = ekin + vtrap * n + ph.c0 * n * n;
+        double Fx = 0, Fy = 0, Fz = 0;
+        if constexpr (C > 1) {
+            const double dxr = rx - ph.center[0], dyr = ry - ph.center[1], dzr = rz - ph.center[2];
+            const double Bx = a.field.bq * dxr + a.field.bb[0];
+            const double By = a.field.bq * dyr + a.field.bb[1];
+            const double Bz = -2 * a.field.bq * dzr + a.field.bb[2];
+            // F = psi^dag f psi through (e_x . f) psi etc. with the same ladder code
+            double tr[C], ti[C];
+            spin_dot<double, C>(1, 0, 0, pr, pi, tr, ti);
+            for (int c = 0; c < C; ++c) Fx += pr[c] * tr[c] + pi[c] * ti[c];
+            spin_dot<double, C>(0, 1, 0, pr, pi, tr, ti);
+            for (int c = 0; c < C; ++c) Fy += pr[c] * tr[c] + pi[c] * ti[c];
+            spin_dot<double, C>(0, 0, 1, pr, pi, tr, ti);
+            for (int c = 0; c < C; ++c) Fz += pr[c] * tr[c] + pi[c] * ti[c];
+            const double sgn = (C == 3) ? 1.0 : -1.0;
+            const double f2 = Fx * Fx + Fy * Fy + Fz * Fz;
+            const double bf = sgn * (Bx * Fx + By * Fy + Bz * Fz);
+            e += 0.5 * ph.c1 * f2 + bf;
+            eh += ph.c1 * f2 + bf;
+            if constexpr (C == 5) {
+                const double thr = 2 * (pr[0] * pr[4] - pi[0] * pi[4]) - 2 * (pr[1] * pr[3] - pi[1] * pi[3]) +
+                                   (pr[2] * pr[2] - pi[2] * pi[2]);
+                const double thi = 2 * (pr[0] * pi[4] + pi[0] * pr[4]) - 2 * (pr[1] * pi[3] + pi[1] * pr[3]) +
+                                   2 * pr[2] * pi[2];
+                const double th2 = thr * thr + thi * thi;
+                e += ph.c2 / 10.0 * th2;
+                eh += ph.c2 / 5.0 * th2;
+            }
+            if (a.quad) {
+                const double Qx = a.field.bqq * dxr + a.field.bbq[0];
+                const double Qy = a.field.bqq * dyr + a.field.bbq[1];
+                const double Qz = -2 * a.field.bqq * dzr + a.field.bbq[2];
+                spin_dot<double, C>(Qx, Qy, Qz, pr, pi, tr, ti);
+                double m2 = 0;
+                for (int c = 0; c < C; ++c) m2 += tr[c] * tr[c] + ti[c] * ti[c];
+                e += ph.quad_sign * m2;
+                eh += ph.quad_sign * m2;
+            }
+            constexpr int F = (C - 1) / 2;
+            double qz = 0;
+            for (int c = 0; c < C; ++c) qz += (double)((F - c) * (F - c)) * (pr[c] * pr[c] + pi[c] * pi[c]);
+            e += ph.q * qz;
+            eh += ph.q * qz;
+        }
+        const double dv = a.tb.dv[v];
+        acc[OBS_NORM] = n * dv;
+        acc[OBS_ENERGY] = e * dv;
+        acc[OBS_EXPECT_H] = eh * dv;
+        acc[OBS_MX] = Fx * dv;
+        acc[OBS_MY] = Fy * dv;
+        acc[OBS_MZ] = Fz * dv;
+        acc[OBS_MAXDENS] = n;
+        acc[OBS_NONFINITE] = (isfinite(n) && isfinite(e)) ? 0.0 : 1.0;
+    }
+    __shared__ double red[4][OBS_COUNT];
+#pragma unroll
+    for (int i = 0; i < OBS_COUNT; ++i) {
+        const double r = (i == OBS_MAXDENS) ? wave_max(acc[i]) : wave_sum(acc[i]);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][i] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < OBS_COUNT) {
+        const int i = threadIdx.x;
+        double r;
+        if (i == OBS_MAXDENS)
+            r = fmax(fmax(red[0][i], red[1][i]), fmax(red[2][i], red[3][i]));
+        else
+            r = (red[0][i] + red[1][i]) + (red[2][i] + red[3][i]);
+        a.partials[(long long)blockIdx.x * OBS_COUNT + i] = r;
+    }
+}
+
+// --------------------------------------------------------------------- small kernels --
+
+__global__ __launch_bounds__(256) void k_finalize(const double *partials, int n, int width, int maxcol, double *out)
+{
+    __shared__ double red[256];
+    for (int col = 0; col < width; ++col) {
+        const bool is_max = col == maxcol;
+        double acc = is_max ? 0.0 : 0.0;
+        for (int i = threadIdx.x; i < n; i += 256) {
+            const double v = partials[(long long)i * width + col];
+            acc = is_max ? fmax(acc, v) : acc + v;
+        }
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s)
+                red[threadIdx.x] = is_max ? fmax(red[threadIdx.x], red[threadIdx.x + s])
+                                          : red[threadIdx.x] + red[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[col] = red[0];
+        __syncthreads();
+    }
+}
+
+__global__ void k_norm_to_scale(const double *in, double target, double *out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = sqrt(target / in[0]);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_scale(vec2_t<R> *psi, long long n, double factor, const double *dfactor)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const R f = (R)(dfactor ? *dfactor : factor);
+    vec2_t<R> o = psi[i];
+    o.x *= f;
+    o.y *= f;
+    psi[i] = o;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_phase_copy(const vec2_t<R> *src, vec2_t<R> *dst, long long n, double pre,
+                                                     double pim)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const vec2_t<R> s = src[i];
+    vec2_t<R> o;
+    o.x = (R)(pre * (double)s.x - pim * (double)s.y);
+    o.y = (R)(pre * (double)s.y + pim * (double)s.x);
+    dst[i] = o;
+}
+
+// ABI index (c, z, y, x, v) <-> device index (c, z, y, v, x)
+template <typename R, bool PACK>
+__global__ __launch_bounds__(256) void k_pack(Geometry g, int ncomp, const double2 *abi_in, double2 *abi_out,
+                                              const vec2_t<R> *dev_in, vec2_t<R> *dev_out)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long total = (long long)ncomp * g.nz * g.ny * g.nx * g.V;
+    if (i >= total) return;
+    long long r = i;
+    const int v = (int)(r % g.V);
+    r /= g.V;
+    const int x = (int)(r % g.nx);
+    r /= g.nx;
+    const int y = (int)(r % g.ny);
+    r /= g.ny;
+    const int z = (int)(r % g.nz);
+    const int c = (int)(r / g.nz);
+    const long long d = line_index(g, c, z, y, v) + x;
+    if constexpr (PACK) {
+        const double2 s = abi_in[i];
+        vec2_t<R> o;
+        o.x = (R)s.x;
+        o.y = (R)s.y;
+        dev_out[d] = o;
+    } else {
+        const vec2_t<R> s = dev_in[d];
+        abi_out[i] = make_double2((double)s.x, (double)s.y);
+    }
+}
+
+// ------------------------------------------------------------------------- launchers --
+
+static inline unsigned grid1(long long n) { return (unsigned)((n + 255) / 256); }
+
+template <typename R, int C, int DEG>
+static hipError_t sweep_mode(int mode, const SweepArgs &a, hipStream_t s)
+{
+    dim3 grid(a.g.blocks), block(256);
+    switch (mode) {
+    case MODE_UPDATE: hipLaunchKernelGGL((k_sweep<R, C, DEG, MODE_UPDATE>), grid, block, 0, s, a); break;
+    case MODE_EULER: hipLaunchKernelGGL((k_sweep<R, C, DEG, MODE_EULER>), grid, block, 0, s, a); break;
+    case MODE_ITP: hipLaunchKernelGGL((k_sweep<R, C, DEG, MODE_ITP>), grid, block, 0, s, a); break;
+    case MODE_HPSI: hipLaunchKernelGGL((k_sweep<R, C, DEG, MODE_HPSI>), grid, block, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename R, int C>
+static hipError_t sweep_deg(int mode, const SweepArgs &a, hipStream_t s)
+{
+    switch (a.g.deg) {
+    case 4: return sweep_mode<R, C, 4>(mode, a, s);
+    case 6: return sweep_mode<R, C, 6>(mode, a, s);
+    case 8: return sweep_mode<R, C, 8>(mode, a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename R>
+static hipError_t sweep_comp(int ncomp, int mode, const SweepArgs &a, hipStream_t s)
+{
+    switch (ncomp) {
+    case 1: return sweep_deg<R, 1>(mode, a, s);
+    case 3: return sweep_deg<R, 3>(mode, a, s);
+    case 5: return sweep_deg<R, 5>(mode, a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_sweep(int fp64, int ncomp, int mode, const SweepArgs &a, hipStream_t s)
+{
+    return fp64 ? sweep_comp<double>(ncomp, mode, a, s) : sweep_comp<float>(ncomp, mode, a, s);
+}
+
+template <typename R, int C>
+static hipError_t obs_deg(const ObsArgs &a, hipStream_t s)
+{
+    dim3 grid(a.g.blocks), block(256);
+    switch (a.g.deg) {
+    case 4: hipLaunchKernelGGL((k_observables<R, C, 4>), grid, block, 0, s, a); break;
+    case 6: hipLaunchKernelGGL((k_observables<R, C, 6>), grid, block, 0, s, a); break;
+    case 8: hipLaunchKernelGGL((k_observables<R, C, 8>), grid, block, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename R>
+static hipError_t obs_comp(int ncomp, const ObsArgs &a, hipStream_t s)
+{
+    switch (ncomp) {
+    case 1: return obs_deg<R, 1>(a, s);
+    case 3: return obs_deg<R, 3>(a, s);
+    case 5: return obs_deg<R, 5>(a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_observables(int fp64, int ncomp, const ObsArgs &a, hipStream_t s)
+{
+    return fp64 ? obs_comp<double>(ncomp, a, s) : obs_comp<float>(ncomp, a, s);
+}
+
+hipError_t launch_finalize(const double *partials, int n, int width, int maxcol, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, partials, n, width, maxcol, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_norm_to_scale(const double *in, double target, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_norm_to_scale, dim3(1), dim3(64), 0, s, in, target, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale(int fp64, void *psi, long long nelem, double factor, const double *dfactor, hipStream_t s)
+{
+    if (nelem <= 0) return hipSuccess;
+    if (fp64)
+        hipLaunchKernelGGL(k_scale<double>, dim3(grid1(nelem)), dim3(256), 0, s, (double2 *)psi, nelem, factor, dfactor);
+    else
+        hipLaunchKernelGGL(k_scale<float>, dim3(grid1(nelem)), dim3(256), 0, s, (float2 *)psi, nelem, factor, dfactor);
+    return hipGetLastError();
+}
+
+hipError_t launch_phase_copy(int fp64, const void *src, void *dst, long long nelem, double pre, double pim,
+                             hipStream_t s)
+{
+    if (nelem <= 0) return hipSuccess;
+    if (fp64)
+        hipLaunchKernelGGL(k_phase_copy<double>, dim3(grid1(nelem)), dim3(256), 0, s, (const double2 *)src,
+                           (double2 *)dst, nelem, pre, pim);
+    else
+        hipLaunchKernelGGL(k_phase_copy<float>, dim3(grid1(nelem)), dim3(256), 0, s, (const float2 *)src,
+                           (float2 *)dst, nelem, pre, pim);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(int fp64, const Geometry &g, int ncomp, const double *abi, void *dev, hipStream_t s)
+{
+    const long long total = (long long)ncomp * g.nz * g.ny * g.nx * g.V;
+    if (fp64)
+        hipLaunchKernelGGL((k_pack<double, true>), dim3(grid1(total)), dim3(256), 0, s, g, ncomp,
+                           (const double2 *)abi, nullptr, nullptr, (double2 *)dev);
+    else
+        hipLaunchKernelGGL((k_pack<float, true>), dim3(grid1(total)), dim3(256), 0, s, g, ncomp,
+                           (const double2 *)abi, nullptr, nullptr, (float2 *)dev);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpack(int fp64, const Geometry &g, int ncomp, const void *dev, double *abi, hipStream_t s)
+{
+    const long long total = (long long)ncomp * g.nz * g.ny * g.nx * g.V;
+    if (fp64)
+        hipLaunchKernelGGL((k_pack<double, false>), dim3(grid1(total)), dim3(256), 0, s, g, ncomp, nullptr,
+                           (double2 *)abi, (const double2 *)dev, nullptr);
+    else
+        hipLaunchKernelGGL((k_pack<float, false>), dim3(grid1(total)), dim3(256), 0, s, g, ncomp, nullptr,
+                           (double2 *)abi, (const float2 *)dev, nullptr);
+    return hipGetLastError();
+}
+
+}  // namespace gpe

@@ -1,0 +1,280 @@
+"""ctypes binding of libgpe.so, the C ABI in include/gpe.h.
+
+This mirrors, call for call, what the reference's host loop does around its kernels
+(``Spin1GpeBCC/Spin1GpeBCC.cu:1105-1743``): create buffers, upload a state, make the second
+time level (forward Euler), step, run imaginary time with renormalisation, and read
+observables.  States are numpy complex128 arrays shaped (ncomp, nz, ny, nx, V), which is
+exactly the ABI memory layout.
+
+There is no CPU fallback: if the HIP library is missing or no device is present, the
+constructor raises.
+"""
+import ctypes
+import os
+from dataclasses import dataclass, field
+from typing import Sequence, Tuple
+
+import numpy as np
+
+from . import lattice as _lattice
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgpe.so")
+
+FP32, FP64 = 0, 1
+LEAPFROG3, STAGGERED = 0, 1
+START_EULER, START_PHASE, START_COPY = 0, 1, 2
+ERRORS = {-22: "EINVAL", -12: "ENOMEM", -5: "EDEVICE", -71: "ESTATE", -34: "ERANGE"}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [
+        ("lattice", ctypes.c_int), ("ncomp", ctypes.c_int), ("precision", ctypes.c_int), ("scheme", ctypes.c_int),
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("block_scale", ctypes.c_double), ("p0", ctypes.c_double * 3), ("dt", ctypes.c_double),
+        ("trap", ctypes.c_double * 3), ("trap_offset", ctypes.c_double), ("trap_off_time", ctypes.c_double),
+        ("c0", ctypes.c_double), ("c1", ctypes.c_double), ("c2", ctypes.c_double), ("alpha", ctypes.c_double),
+        ("quad_sign", ctypes.c_double), ("q", ctypes.c_double), ("center", ctypes.c_double * 3),
+    ]
+
+
+class _Ramp(ctypes.Structure):
+    _fields_ = [
+        ("nq", ctypes.c_int), ("q_val", _dp), ("q_dur", _dp), ("q_type", _ip),
+        ("nb", ctypes.c_int), ("b_val", _dp), ("b_dur", _dp), ("b_type", _ip),
+        ("tau", ctypes.c_double), ("time_scale", ctypes.c_double),
+        ("scale_q", ctypes.c_double), ("scale_b", ctypes.c_double),
+        ("qscale_q", ctypes.c_double), ("qscale_b", ctypes.c_double),
+    ]
+
+
+class _Obs(ctypes.Structure):
+    _fields_ = [
+        ("norm", ctypes.c_double), ("energy", ctypes.c_double), ("expect_h", ctypes.c_double),
+        ("magnetization", ctypes.c_double * 3), ("max_density", ctypes.c_double), ("time", ctypes.c_double),
+        ("finite", ctypes.c_int),
+    ]
+
+
+class GpeError(RuntimeError):
+    pass
+
+
+_LIB = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libgpe.so once; raises GpeError if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise GpeError("libgpe.so not built at %s: run `python -m spacetime_amd.build`" % path)
+    lib = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    lib.gpe_create.argtypes = [ctypes.POINTER(_Config), ctypes.POINTER(vp)]
+    lib.gpe_destroy.argtypes = [vp]
+    lib.gpe_last_error.argtypes = [vp]
+    lib.gpe_last_error.restype = ctypes.c_char_p
+    lib.gpe_lattice_values.argtypes = [ctypes.c_int]
+    for name in ("gpe_set_state", "gpe_set_state_device"):
+        getattr(lib, name).argtypes = [vp, ctypes.c_int, vp, ctypes.c_double]
+    for name in ("gpe_get_state", "gpe_get_state_device"):
+        getattr(lib, name).argtypes = [vp, ctypes.c_int, vp]
+    lib.gpe_set_field.argtypes = [vp, ctypes.c_double, _dp, ctypes.c_double, _dp]
+    lib.gpe_set_ramp.argtypes = [vp, ctypes.POINTER(_Ramp)]
+    lib.gpe_start.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+    lib.gpe_step.argtypes = [vp, ctypes.c_int]
+    lib.gpe_itp.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+    lib.gpe_observables.argtypes = [vp, ctypes.c_int, ctypes.POINTER(_Obs)]
+    lib.gpe_scale.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+    lib.gpe_sync.argtypes = [vp]
+    lib.gpe_time.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_longlong)]
+    lib.gpe_last_step_timing.argtypes = [vp, _dp, _dp]
+    _LIB = lib
+    return lib
+
+
+@dataclass
+class Config:
+    """Mirror of gpe_config (include/gpe.h)."""
+    lattice: str = "bcc"
+    ncomp: int = 1
+    precision: int = FP64
+    scheme: int = LEAPFROG3
+    dims: Tuple[int, int, int] = (8, 8, 8)
+    block_scale: float = 0.1
+    p0: Sequence[float] = (0.0, 0.0, 0.0)
+    dt: float = 1e-4
+    trap: Sequence[float] = (1.0, 1.0, 1.0)
+    trap_offset: float = 0.0
+    trap_off_time: float = 1e300
+    c0: float = 0.0
+    c1: float = 0.0
+    c2: float = 0.0
+    alpha: float = 0.0
+    quad_sign: float = None
+    q: float = 0.0
+    center: Sequence[float] = (0.0, 0.0, 0.0)
+
+    @property
+    def V(self):
+        return _lattice.get(self.lattice).V
+
+    @property
+    def shape(self):
+        nx, ny, nz = self.dims
+        return (self.ncomp, nz, ny, nx, self.V)
+
+    def to_c(self):
+        c = _Config()
+        c.lattice = _lattice.LATTICE_IDS[self.lattice]
+        c.ncomp, c.precision, c.scheme = self.ncomp, self.precision, self.scheme
+        c.nx, c.ny, c.nz = self.dims
+        c.block_scale, c.dt = self.block_scale, self.dt
+        c.p0[:] = list(self.p0)
+        c.trap[:] = list(self.trap)
+        c.trap_offset, c.trap_off_time = self.trap_offset, self.trap_off_time
+        c.c0, c.c1, c.c2, c.alpha, c.q = self.c0, self.c1, self.c2, self.alpha, self.q
+        c.quad_sign = (1.0 if self.ncomp == 3 else -1.0) if self.quad_sign is None else self.quad_sign
+        c.center[:] = list(self.center)
+        return c
+
+    @classmethod
+    def centred_box(cls, lattice, dims, cell_edge, **kw):
+        """Box of dims blocks of physical edge cell_edge centred on the origin."""
+        lat = _lattice.get(lattice)
+        s = cell_edge / lat.block_width
+        p0 = tuple(-0.5 * cell_edge * d for d in dims)
+        return cls(lattice=lattice, dims=tuple(dims), block_scale=s, p0=p0, **kw)
+
+
+@dataclass
+class Observables:
+    norm: float
+    energy: float
+    expect_h: float
+    magnetization: np.ndarray
+    max_density: float
+    time: float
+    finite: bool
+
+
+class Stepper:
+    """One simulation context on the current HIP device (gpe_create ... gpe_destroy)."""
+
+    def __init__(self, config: Config):
+        self.lib = load_library()
+        self.config = config
+        self._cfg = config.to_c()
+        h = ctypes.c_void_p()
+        rc = self.lib.gpe_create(ctypes.byref(self._cfg), ctypes.byref(h))
+        if rc != 0:
+            raise GpeError("gpe_create failed: %s (%d)" % (ERRORS.get(rc, "?"), rc))
+        self._h = h
+        self._keep = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.gpe_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.gpe_last_error(self._h)
+            raise GpeError("%s failed: %s (%d): %s" % (what, ERRORS.get(rc, "?"), rc, msg.decode() if msg else ""))
+
+    # -- state ---------------------------------------------------------------------------------
+    def set_state(self, psi, level=0, t=0.0):
+        psi = np.ascontiguousarray(psi, dtype=np.complex128)
+        if psi.shape != self.config.shape:
+            raise ValueError("state shape %s != %s" % (psi.shape, self.config.shape))
+        self._check(self.lib.gpe_set_state(self._h, level, psi.ctypes.data, t), "gpe_set_state")
+
+    def get_state(self, level=0):
+        out = np.empty(self.config.shape, dtype=np.complex128)
+        self._check(self.lib.gpe_get_state(self._h, level, out.ctypes.data), "gpe_get_state")
+        return out
+
+    def set_state_device(self, ptr, level=0, t=0.0):
+        self._check(self.lib.gpe_set_state_device(self._h, level, ctypes.c_void_p(ptr), t), "gpe_set_state_device")
+
+    def get_state_device(self, ptr, level=0):
+        self._check(self.lib.gpe_get_state_device(self._h, level, ctypes.c_void_p(ptr)), "gpe_get_state_device")
+
+    # -- fields --------------------------------------------------------------------------------
+    def set_field(self, bq=0.0, bb=(0.0, 0.0, 0.0), bq_quad=0.0, bb_quad=(0.0, 0.0, 0.0)):
+        b = (ctypes.c_double * 3)(*bb)
+        bq3 = (ctypes.c_double * 3)(*bb_quad)
+        self._check(self.lib.gpe_set_field(self._h, bq, b, bq_quad, bq3), "gpe_set_field")
+
+    def set_ramp(self, q_segments=(), b_segments=(), tau=1.0, time_scale=1.0, scale_q=1.0, scale_b=1.0,
+                 qscale_q=0.0, qscale_b=0.0):
+        """Segments are (value, duration, type); bias values are 3-vectors (gpe_ramp)."""
+        qv = np.array([s[0] for s in q_segments] or [0.0], dtype=np.float64)
+        qd = np.array([s[1] for s in q_segments] or [0.0], dtype=np.float64)
+        qt = np.array([s[2] for s in q_segments] or [0], dtype=np.int32)
+        bv = np.array([list(s[0]) for s in b_segments] or [[0.0] * 3], dtype=np.float64).reshape(-1)
+        bd = np.array([s[1] for s in b_segments] or [0.0], dtype=np.float64)
+        bt = np.array([s[2] for s in b_segments] or [0], dtype=np.int32)
+        r = _Ramp()
+        r.nq, r.q_val, r.q_dur, r.q_type = len(q_segments), qv.ctypes.data_as(_dp), qd.ctypes.data_as(_dp), \
+            qt.ctypes.data_as(_ip)
+        r.nb, r.b_val, r.b_dur, r.b_type = len(b_segments), bv.ctypes.data_as(_dp), bd.ctypes.data_as(_dp), \
+            bt.ctypes.data_as(_ip)
+        r.tau, r.time_scale = tau, time_scale
+        r.scale_q, r.scale_b, r.qscale_q, r.qscale_b = scale_q, scale_b, qscale_q, qscale_b
+        self._check(self.lib.gpe_set_ramp(self._h, ctypes.byref(r)), "gpe_set_ramp")
+
+    def clear_ramp(self):
+        self._check(self.lib.gpe_set_ramp(self._h, None), "gpe_set_ramp")
+
+    # -- time stepping -------------------------------------------------------------------------
+    def start(self, mode=START_EULER, mu=0.0):
+        self._check(self.lib.gpe_start(self._h, mode, mu), "gpe_start")
+
+    def step(self, n):
+        self._check(self.lib.gpe_step(self._h, int(n)), "gpe_step")
+
+    def itp(self, n, dt):
+        self._check(self.lib.gpe_itp(self._h, int(n), dt), "gpe_itp")
+
+    def scale(self, factor, level=0):
+        self._check(self.lib.gpe_scale(self._h, level, factor), "gpe_scale")
+
+    def sync(self):
+        self._check(self.lib.gpe_sync(self._h), "gpe_sync")
+
+    def observables(self, level=0, check=True):
+        o = _Obs()
+        rc = self.lib.gpe_observables(self._h, level, ctypes.byref(o))
+        if check or rc not in (0, -34):
+            self._check(rc, "gpe_observables")
+        return Observables(o.norm, o.energy, o.expect_h, np.array(o.magnetization[:]), o.max_density, o.time,
+                           bool(o.finite))
+
+    def time(self):
+        t = ctypes.c_double()
+        n = ctypes.c_longlong()
+        self._check(self.lib.gpe_time(self._h, ctypes.byref(t), ctypes.byref(n)), "gpe_time")
+        return t.value, n.value
+
+    def last_step_timing(self):
+        ms = ctypes.c_double()
+        by = ctypes.c_double()
+        self._check(self.lib.gpe_last_step_timing(self._h, ctypes.byref(ms), ctypes.byref(by)), "timing")
+        return ms.value, by.value

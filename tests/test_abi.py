@@ -1,0 +1,58 @@
+"""The C-ABI library loads on a machine without a GPU and exports every entry point the
+public headers declare; host-only argument validation works without touching a device."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = [os.path.join(ROOT, "include", f) for f in sorted(os.listdir(os.path.join(ROOT, "include")))
+           if f.endswith(".h")]
+LIB = os.path.join(ROOT, "spacetime_amd", "libgpe.so")
+
+
+def declared_functions():
+    names = set()
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(gpe_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(LIB):
+        from spacetime_amd import build
+        build.build()
+    return ctypes.CDLL(LIB)
+
+
+def test_headers_declare_the_core_api():
+    names = declared_functions()
+    for n in ("gpe_create", "gpe_destroy", "gpe_set_state", "gpe_get_state", "gpe_step", "gpe_itp",
+              "gpe_observables", "gpe_start", "gpe_set_ramp", "gpe_set_field", "gpe_last_error"):
+        assert n in names
+
+
+def test_every_declared_symbol_is_exported(lib):
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_lattice_values_host_only(lib):
+    assert [lib.gpe_lattice_values(i) for i in range(5)] == [1, 12, 12, 46, 136]
+    assert lib.gpe_lattice_values(7) == -22
+
+
+def test_create_rejects_bad_config_without_device(lib):
+    from spacetime_amd import gpe
+    cfg = gpe.Config(lattice="bcc", dims=(0, 4, 4)).to_c()
+    h = ctypes.c_void_p()
+    lib.gpe_create.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    assert lib.gpe_create(ctypes.byref(cfg), ctypes.byref(h)) == -22
+    cfg = gpe.Config(lattice="bcc", ncomp=2).to_c()
+    assert lib.gpe_create(ctypes.byref(cfg), ctypes.byref(h)) == -22
+    assert not h.value

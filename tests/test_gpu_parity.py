@@ -1,0 +1,209 @@
+"""HIP path (libgpe.so through the C ABI) vs the FP64 CPU oracle on the same seeded inputs.
+
+Tolerances: FP64 form 1e-12 relative L2 (different summation order and FMA contraction
+only); FP32 form 2e-5 relative L2 over a few tens of sweeps on random states.
+"""
+import numpy as np
+import pytest
+
+from helpers import normalized, oracle_params, random_state, rel_l2, with_field
+
+pytestmark = pytest.mark.gpu
+
+gpe = pytest.importorskip("spacetime_amd.gpe")
+
+FULL = dict(trap=(1.0, 0.9, 1.2), trap_offset=0.5, c0=60.0, c1=8.0, c2=-6.0, q=0.3, center=(0.02, -0.01, 0.03))
+FIELD = dict(bq=0.4, bb=(0.1, -0.05, 0.3), bq_quad=0.2, bb_quad=(0.05, 0.02, -0.1))
+
+
+def _config(lattice, ncomp, precision, dims, dt=2e-4, scheme=0, cell=0.6, **kw):
+    phys = dict(FULL)
+    phys.update(kw)
+    return gpe.Config.centred_box(lattice, dims, cell, ncomp=ncomp, precision=precision, dt=dt, scheme=scheme, **phys)
+
+
+def _run_both(oracle, cfg, nsteps, field=None, seed=1):
+    from spacetime_amd import lattice
+    lat = lattice.get(cfg.lattice)
+    L = oracle.lattice(lat)
+    p = oracle_params(oracle, cfg)
+    if field:
+        with_field(p, **field)
+    psi0 = random_state(cfg.shape, seed)
+    a = psi0.copy()
+    b = oracle.forward_euler(L, p, a, cfg.dt)
+    oracle.leapfrog3(L, p, a, b, cfg.dt, nsteps, t0=cfg.dt)
+    with gpe.Stepper(cfg) as st:
+        if field:
+            st.set_field(**field)
+        st.set_state(psi0)
+        st.start(gpe.START_EULER)
+        st.step(nsteps)
+        g0, g1 = st.get_state(0), st.get_state(1)
+        t, sweeps = st.time()
+    assert sweeps == nsteps + 1
+    assert t == pytest.approx((nsteps + 1) * cfg.dt)
+    return (g0, g1), (b, a)
+
+
+@pytest.mark.parametrize("lattice_name", ["cube", "bcc", "fcc", "a15", "c15"])
+@pytest.mark.parametrize("ncomp", [1, 3, 5])
+def test_fp64_sweeps_match_oracle(oracle, lattice_name, ncomp):
+    dims = (5, 4, 3) if lattice_name != "c15" else (3, 2, 2)
+    cfg = _config(lattice_name, ncomp, gpe.FP64, dims)
+    (g0, g1), (o0, o1) = _run_both(oracle, cfg, 12, field=FIELD if ncomp > 1 else None)
+    assert rel_l2(g0, o0) < 1e-12
+    assert rel_l2(g1, o1) < 1e-12
+
+
+@pytest.mark.parametrize("lattice_name", ["cube", "bcc", "fcc", "a15", "c15"])
+@pytest.mark.parametrize("ncomp", [1, 3, 5])
+def test_fp32_sweeps_match_oracle(oracle, lattice_name, ncomp):
+    dims = (5, 4, 3) if lattice_name != "c15" else (3, 2, 2)
+    cfg = _config(lattice_name, ncomp, gpe.FP32, dims)
+    (g0, g1), (o0, o1) = _run_both(oracle, cfg, 12, field=FIELD if ncomp > 1 else None)
+    assert rel_l2(g0, o0) < 2e-5
+    assert rel_l2(g1, o1) < 2e-5
+
+
+@pytest.mark.parametrize("precision,tol", [(1, 1e-12), (0, 2e-5)])
+def test_wide_rows_span_several_waves(oracle, precision, tol):
+    """nx > 64 exercises several wave segments per x-line and partial last waves."""
+    cfg = _config("bcc", 3, precision, (70, 3, 2))
+    (g0, g1), (o0, o1) = _run_both(oracle, cfg, 6, field=FIELD)
+    assert rel_l2(g0, o0) < tol and rel_l2(g1, o1) < tol
+
+
+def test_loss_term(oracle):
+    cfg = _config("bcc", 3, gpe.FP64, (4, 3, 3), alpha=5.0)
+    (g0, _), (o0, _) = _run_both(oracle, cfg, 10, field=FIELD)
+    assert rel_l2(g0, o0) < 1e-12
+
+
+def test_staggered_scalar_with_phase_start(oracle):
+    """Scalar staggered scheme started with odd = exp(i mu dt/2) even (GPBlocks.cpp:516-523)."""
+    from spacetime_amd import lattice
+    cfg = _config("bcc", 1, gpe.FP64, (5, 5, 4), dt=1e-3, scheme=gpe.STAGGERED, c1=0.0, c2=0.0, q=0.0)
+    lat = lattice.get("bcc")
+    L = oracle.lattice(lat)
+    p = oracle_params(oracle, cfg)
+    even = random_state(cfg.shape, 5)
+    mu = 3.7
+    odd = np.exp(1j * mu * cfg.dt / 2) * even
+    e, o = even.copy(), odd.copy()
+    oracle.staggered(L, p, e, o, cfg.dt, 15)
+    with gpe.Stepper(cfg) as st:
+        st.set_state(even)
+        st.start(gpe.START_PHASE, mu)
+        st.step(15)
+        g_even, g_odd = st.get_state(0), st.get_state(1)
+        t, sweeps = st.time()
+    assert sweeps == 30 and t == pytest.approx(15 * cfg.dt)
+    assert rel_l2(g_even, e) < 1e-12
+    assert rel_l2(g_odd, o) < 1e-12
+
+
+@pytest.mark.parametrize("ncomp", [1, 3, 5])
+def test_itp_matches_oracle(oracle, ncomp):
+    from spacetime_amd import lattice
+    cfg = _config("bcc", ncomp, gpe.FP64, (4, 4, 3))
+    L = oracle.lattice(lattice.get("bcc"))
+    p = oracle_params(oracle, cfg)
+    psi = random_state(cfg.shape, 8)
+    ref = psi.copy()
+    oracle.normalize(L, p, ref)
+    for _ in range(25):
+        oracle.itp_step(L, p, ref, 1e-3)
+    with gpe.Stepper(cfg) as st:
+        st.set_state(psi)
+        st.itp(25, 1e-3)
+        got = st.get_state(0)
+        obs = st.observables(0)
+    assert obs.norm == pytest.approx(1.0, rel=1e-12)
+    assert rel_l2(got, ref) < 1e-12
+
+
+@pytest.mark.parametrize("lattice_name,ncomp,precision", [
+    ("bcc", 1, 1), ("bcc", 3, 1), ("bcc", 5, 1), ("fcc", 3, 1), ("a15", 5, 1), ("bcc", 3, 0)])
+def test_observables_match_oracle(oracle, lattice_name, ncomp, precision):
+    from spacetime_amd import lattice
+    cfg = _config(lattice_name, ncomp, precision, (4, 3, 3))
+    L = oracle.lattice(lattice.get(lattice_name))
+    p = with_field(oracle_params(oracle, cfg), **FIELD)
+    psi = random_state(cfg.shape, 12)
+    with gpe.Stepper(cfg) as st:
+        st.set_field(**FIELD)
+        st.set_state(psi)
+        o = st.observables(0)
+        stored = st.get_state(0)
+    tol = 1e-12 if precision == 1 else 1e-6
+    # FP32 storage rounds the state; compare against the oracle on the stored values
+    assert o.norm == pytest.approx(oracle.norm(L, p, stored), rel=tol)
+    assert o.energy == pytest.approx(oracle.energy(L, p, stored), rel=tol)
+    assert o.expect_h == pytest.approx(oracle.expect_h(L, p, stored), rel=tol)
+    m = oracle.magnetization(L, p, stored)
+    np.testing.assert_allclose(o.magnetization, m, rtol=tol, atol=tol * abs(o.norm))
+    assert o.max_density == pytest.approx(np.max(np.sum(np.abs(stored) ** 2, axis=0)), rel=1e-12)
+    assert o.finite
+    if precision == 0:
+        assert rel_l2(stored, psi) < 1e-7
+
+
+def test_ramp_matches_oracle(oracle):
+    """16-knot piecewise-linear Bz ramp plus a constant quadrupole gradient."""
+    from spacetime_amd import lattice
+    cfg = _config("bcc", 3, gpe.FP64, (4, 4, 3), q=0.1)
+    rng = np.random.default_rng(16)
+    knots = rng.uniform(-0.1, 0.1, 16)
+    T = 20 * cfg.dt
+    b_segments = [((0, 0, knots[0]), 0.0, 0)] + [((0, 0, k), T / 15, 1) for k in knots[1:]]
+    q_segments = [(4e-3, 1e9, 0)]
+    L = oracle.lattice(lattice.get("bcc"))
+    p = oracle_params(oracle, cfg)
+    r = oracle.ramp(q_segments=q_segments, b_segments=b_segments)
+    psi0 = random_state(cfg.shape, 3)
+    a = psi0.copy()
+    # forward Euler at t = 0 with the ramp field at t = 0
+    bq0, bb0 = oracle.ramp_signal(r, 0.0)
+    with_field(p, bq=bq0, bb=bb0)
+    b = oracle.forward_euler(L, p, a, cfg.dt)
+    oracle.leapfrog3(L, p, a, b, cfg.dt, 22, t0=cfg.dt, ramp=r)
+    with gpe.Stepper(cfg) as st:
+        st.set_ramp(q_segments=q_segments, b_segments=b_segments)
+        st.set_state(psi0)
+        st.start(gpe.START_EULER)
+        st.step(22)
+        g0 = st.get_state(0)
+    assert rel_l2(g0, b) < 1e-12
+
+
+def test_fp32_tracks_fp64_over_many_steps():
+    """The FP32 form stays within 1e-4 relative L2 of the FP64 form (small grid)."""
+    from spacetime_amd import lattice  # noqa: F401
+    cfgs = [_config("bcc", 3, prec, (16, 16, 16), dt=1e-4, cell=0.5, c0=1000.0, c1=50.0, c2=0.0) for prec in (0, 1)]
+    psi0 = normalized(random_state(cfgs[0].shape, 4), cfgs[0])
+    out = []
+    for cfg in cfgs:
+        with gpe.Stepper(cfg) as st:
+            st.set_field(bq=4e-3, bb=(0, 0, 0.05))
+            st.set_state(psi0)
+            st.start(gpe.START_EULER)
+            st.step(2000)
+            out.append(st.get_state(0))
+    assert rel_l2(out[0], out[1]) < 1e-4
+
+
+def test_errors_are_reported():
+    cfg = _config("bcc", 3, gpe.FP64, (4, 4, 4))
+    with gpe.Stepper(cfg) as st:
+        with pytest.raises(gpe.GpeError, match="ESTATE"):
+            st.step(3)  # no start yet
+        with pytest.raises(ValueError):
+            st.set_state(np.zeros((3, 1, 1, 1, 12)))
+        bad = np.zeros(cfg.shape, dtype=np.complex128)
+        bad[0, 0, 0, 0, 0] = np.nan
+        st.set_state(bad)
+        o = st.observables(0, check=False)
+        assert not o.finite
+    with pytest.raises(gpe.GpeError, match="EINVAL"):
+        gpe.Stepper(gpe.Config(lattice="bcc", dims=(0, 4, 4)))
