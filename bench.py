@@ -38,6 +38,32 @@ def _sync():
         pass
 
 
+def copy_bandwidth(nbytes=2 << 30, reps=20):
+    """Device copy bandwidth (read + write bytes / time) of a plain torch copy kernel."""
+    try:
+        import torch
+        if not torch.cuda.is_available():
+            return None
+        n = nbytes // 4
+        a = torch.empty(n, dtype=torch.float32, device="cuda").uniform_()
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            b.copy_(a)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        del a, b
+        torch.cuda.empty_cache()
+        return 2.0 * nbytes / (ms * 1e-3) / 1e9
+    except Exception:
+        return None
+
+
 def workload_spin1(seed=2024):
     from spacetime_amd import gpe
     cells = (128, 128, 128)
@@ -140,7 +166,7 @@ def cpu_baseline(cfg, psi0, log, budget_s=15.0):
     t0 = time.perf_counter()
     o.update(L, p, b, a, coef)
     t1 = time.perf_counter() - t0
-    n = max(1, min(50, int(budget_s / max(t1, 1e-3))))
+    n = max(1, min(400, int(budget_s / max(t1, 1e-3))))
     t0 = time.perf_counter()
     for k in range(n):
         o.update(L, p, b, a, coef)
@@ -179,6 +205,9 @@ def main():
     nodes = int(np.prod(cfg.dims) * cfg.V)
     info.update(unknowns=nodes, lattice_points=int(2 * np.prod(cfg.dims)))
     log("workload %s: %d unknowns" % (info["workload"], nodes))
+    copy_gbs = copy_bandwidth()
+    if copy_gbs:
+        log("device copy bandwidth (torch copy_, 2 GiB): %.1f GB/s" % copy_gbs)
     psi0 = prepare_state(cfg, knots, args.itp_steps, args.itp_dt, log)
 
     r32, psi32 = run_form(dataclasses.replace(cfg, precision=gpe.FP32), psi0, knots, args.warmup, args.steps, log)
@@ -220,6 +249,8 @@ def main():
             "frac": r32["achieved_gbs"] / PEAK_HBM_GBS,
             "traffic": None,
             "kernel_ms_per_sweep": r32["kernel_ms_per_sweep"],
+            "copy_gbs": copy_gbs,
+            "frac_of_copy": (r32["achieved_gbs"] / copy_gbs) if copy_gbs else None,
             "algorithmic_bytes_per_sweep": r32["bytes_per_sweep"],
         },
         "cpu_baseline": cpu,

@@ -10,7 +10,8 @@
  * State layout across the ABI (host or device arrays, complex interleaved re/im):
  *   psi[2 * (((c * nz + z) * ny + y) * nx + x) * V + v) + {0, 1}]
  * component c (m = F, F-1, ..., -F), block (x, y, z), unknown v of the block.  The device
- * keeps its own layout; the ABI layout is independent of it.
+ * keeps its own layout (x-fastest lines per (component, z, y, unknown); gpe_internal.h);
+ * the ABI layout is independent of it.
  *
  * Threading: a context owns one HIP stream; calls on one context are stream-ordered and
  * not thread-safe; distinct contexts may be driven from distinct threads.  Calls that
@@ -157,6 +158,40 @@ int gpe_time(const gpe_ctx *ctx, double *t, long long *sweeps);
  * HIP events on the context's stream; bytes_per_sweep gets the algorithmic bytes of one
  * sweep (3 touches x ncomp x 2 x sizeof(real) per unknown). */
 int gpe_last_step_timing(const gpe_ctx *ctx, double *ms_per_sweep, double *bytes_per_sweep);
+
+/* ---------------------------------------------------------------- discrete adjoint ---- */
+/* No counterpart in the reference: gradients of a final-state objective with respect to
+ * the field-ramp segment values and the trap frequencies, by the discrete adjoint of the
+ * 3-level leapfrog with checkpoints every K sweeps and recomputation (FP64 accumulation).
+ * Starts from level 0 (call right after gpe_set_state); requires GPE_SCHEME_LEAPFROG3,
+ * alpha = 0 and no ramp on the quadratic field (qscale_q = qscale_b = 0).  On return the
+ * context holds psi_N (level 0) and psi_{N-1} (level 1). */
+#define GPE_OBJ_OVERLAP 0         /* J = |<target|psi_N>|^2, <a|b> = sum dv conj(a) b */
+#define GPE_OBJ_ENERGY 1          /* J = E[psi_N] (GP energy functional)              */
+#define GPE_OBJ_MAGNETIZATION_Z 2 /* J = sum dv psi^dag f_z psi                         */
+
+typedef struct {
+    int nsweeps;          /* N >= 1: forward Euler + (N-1) leapfrog sweeps                */
+    int checkpoint_every; /* K >= 2: a (psi_jK, psi_jK+1) pair is kept every K sweeps     */
+    int objective;        /* GPE_OBJ_*                                                    */
+    const double *target; /* GPE_OBJ_OVERLAP: host state, ABI layout; else NULL           */
+} gpe_adjoint_spec;
+
+typedef struct {
+    double objective;          /* J                                                      */
+    double grad_trap[3];       /* dJ / d trap[a]                                         */
+    double grad_static_bq;     /* dJ / d bq       (static field, gpe_set_field)          */
+    double grad_static_bb[3];  /* dJ / d bb[a]    (static field)                         */
+    double *grad_ramp_q;       /* caller array [nq] or NULL: dJ / d q_val[j] of the ramp  */
+    double *grad_ramp_b;       /* caller array [nb*3] or NULL: dJ / d b_val[j*3+a]        */
+    double forward_ms, backward_ms;
+    double checkpoint_bytes, segment_bytes;
+} gpe_adjoint_result;
+
+int gpe_adjoint(gpe_ctx *ctx, const gpe_adjoint_spec *spec, gpe_adjoint_result *res);
+
+/* Replace the trap frequencies of a context (V = 0.5 sum (trap_a r_a)^2 + offset). */
+int gpe_set_trap(gpe_ctx *ctx, const double trap[3]);
 
 #ifdef __cplusplus
 }

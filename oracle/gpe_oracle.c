@@ -738,3 +738,142 @@ int orc_gpblocks_integrate(const orc_lattice *L, int kappa, double g, const doub
     free(psi0); free(even); free(odd);
     return 0;
 }
+
+/* ------------------------------------------------------------ discrete adjoint ---- */
+
+/* G'(psi)^T mu at one unknown for the local part, in the general spin-F form:
+ *   i [ (V + c0 n) mu + (c1 F + s B).f mu + quad_sign (Q.f)^2 mu + q fz^2 mu ]
+ *   - 2 c0 Im(psi^dag mu) psi - 2 c1 sum_a Im(psi^dag f_a mu) f_a psi
+ *   + i (2 c2/5) u (u^dag mu) - i (c2/5) Theta S conj(mu_rev)    (spin-2, u = S conj(psi_rev))
+ * plus i * kin, supplied by the caller.  Result accumulated into out. */
+static void adj_local(const orc_params *p, const double r[3], double t, const cplx *psi, const cplx *mu,
+                      const cplx *kin, cplx *out)
+{
+    const int C = p->ncomp, F = (C - 1) / 2;
+    cplx h[5];
+    double n = 0;
+    int c, a;
+    for (c = 0; c < C; ++c) n += cnorm2(psi[c]);
+    const double d = trap_potential(p, r, t) + p->c0 * n;
+    for (c = 0; c < C; ++c) h[c] = cadd(kin[c], cscale(d, mu[c]));
+    if (F > 0) {
+        double Fv[3], B[3], W[3];
+        spin_density(F, psi, Fv);
+        field_at(p->bq, p->bb, p->center, r, B);
+        const double s = (F == 1) ? 1.0 : -1.0;
+        for (a = 0; a < 3; ++a) W[a] = p->c1 * Fv[a] + s * B[a];
+        spin_dot_apply(F, W, mu, h);
+        if (p->bq_quad != 0.0 || p->bb_quad[0] != 0.0 || p->bb_quad[1] != 0.0 || p->bb_quad[2] != 0.0) {
+            double Q[3];
+            cplx t1[5] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}}, t2[5] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+            field_at(p->bq_quad, p->bb_quad, p->center, r, Q);
+            spin_dot_apply(F, Q, mu, t1);
+            spin_dot_apply(F, Q, t1, t2);
+            for (c = 0; c < C; ++c) h[c] = cadd(h[c], cscale(p->quad_sign, t2[c]));
+        }
+        for (c = 0; c < C; ++c) {
+            int m = F - c;
+            h[c] = cadd(h[c], cscale(p->q * m * m, mu[c]));
+        }
+    }
+    /* i h */
+    for (c = 0; c < C; ++c) out[c] = cadd(out[c], cx(-h[c].im, h[c].re));
+    /* -2 c0 Im(psi^dag mu) psi */
+    double ima = 0;
+    for (c = 0; c < C; ++c) ima += cmul(cconj(psi[c]), mu[c]).im;
+    for (c = 0; c < C; ++c) out[c] = csub(out[c], cscale(2 * p->c0 * ima, psi[c]));
+    if (F > 0) {
+        double Y[3];
+        for (a = 0; a < 3; ++a) {
+            double e[3] = {0, 0, 0};
+            cplx fm[5] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+            e[a] = 1.0;
+            spin_dot_apply(F, e, mu, fm);
+            double im = 0;
+            for (c = 0; c < C; ++c) im += cmul(cconj(psi[c]), fm[c]).im;
+            Y[a] = im;
+        }
+        cplx yp[5] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+        spin_dot_apply(F, Y, psi, yp);
+        for (c = 0; c < C; ++c) out[c] = csub(out[c], cscale(2 * p->c1, yp[c]));
+        if (F == 2 && p->c2 != 0.0) {
+            cplx u[5], udmu = cx(0, 0), theta = cx(0, 0);
+            for (c = 0; c < C; ++c) {
+                const double sg = (c & 1) ? -1.0 : 1.0;
+                u[c] = cscale(sg, cconj(psi[C - 1 - c]));
+                udmu = cadd(udmu, cmul(cconj(u[c]), mu[c]));
+                theta = cadd(theta, cscale(sg, cmul(psi[c], psi[C - 1 - c])));
+            }
+            for (c = 0; c < C; ++c) {
+                const double sg = (c & 1) ? -1.0 : 1.0;
+                cplx tp = cscale(0.4 * p->c2, cmul(u[c], udmu));                          /* (2c2/5) u (u^dag mu) */
+                cplx tq = cscale(0.2 * p->c2 * sg, cmul(theta, cconj(mu[C - 1 - c]))); /* (c2/5) Theta S mu*_rev */
+                out[c] = cadd(out[c], cx(-tp.im, tp.re));
+                out[c] = cadd(out[c], cx(tq.im, -tq.re));
+            }
+        }
+    }
+}
+
+/* One adjoint sweep over the grid: mu2 += coef G'(psi)^T mu1; sens[7] += plain-inner-product
+ * sensitivities sum Im(conj(mu1) X psi) for X = s f_x, s f_y, s f_z, s (dx fx + dy fy - 2 dz fz),
+ * w_x x^2, w_y y^2, w_z z^2. */
+void orc_adjoint_sweep(const orc_lattice *L, const orc_params *p, double t, double coef, const double *psi_,
+                       const double *mu1_, double *mu2_, double sens[7])
+{
+    const cplx *psi = (const cplx *)psi_, *mu1 = (const cplx *)mu1_;
+    cplx *mu2 = (cplx *)mu2_;
+    const int V = L->V, C = p->ncomp, F = (C - 1) / 2;
+    const size_t nc = ncells(p) * V;
+    const double inv_s2 = 1.0 / (p->block_scale * p->block_scale);
+    int x, y, z, v, c, f, k;
+    for (k = 0; k < 7; ++k) sens[k] = 0;
+    for (z = 0; z < p->nz; ++z)
+        for (y = 0; y < p->ny; ++y)
+            for (x = 0; x < p->nx; ++x)
+                for (v = 0; v < V; ++v) {
+                    const size_t me = ((size_t)(z * p->ny + y) * p->nx + x) * V + v;
+                    cplx ps[5], mu[5], kin[5], out[5];
+                    double r[3];
+                    for (c = 0; c < C; ++c) {
+                        ps[c] = psi[c * nc + me];
+                        mu[c] = mu1[c * nc + me];
+                        kin[c] = cx(0, 0);
+                        out[c] = cx(0, 0);
+                    }
+                    for (f = L->face_start[v]; f < L->face_start[v + 1]; ++f) {
+                        const int ax = x + L->face_d[3 * f], ay = y + L->face_d[3 * f + 1], az = z + L->face_d[3 * f + 2];
+                        const double w = -0.5 * L->face_hodge[f] * inv_s2;
+                        int inside = ax >= 0 && ax < p->nx && ay >= 0 && ay < p->ny && az >= 0 && az < p->nz;
+                        size_t nb = inside ? ((size_t)(az * p->ny + ay) * p->nx + ax) * V + L->face_tgt[f] : 0;
+                        for (c = 0; c < C; ++c) {
+                            cplx o = inside ? mu1[c * nc + nb] : cx(0, 0);
+                            kin[c] = cadd(kin[c], cscale(w, csub(o, mu[c])));
+                        }
+                    }
+                    orc_position(L, p, x, y, z, v, r);
+                    adj_local(p, r, t, ps, mu, kin, out);
+                    for (c = 0; c < C; ++c) mu2[c * nc + me] = cadd(mu2[c * nc + me], cscale(coef, out[c]));
+                    /* sensitivities */
+                    double im_mp = 0;
+                    for (c = 0; c < C; ++c) im_mp += cmul(cconj(mu[c]), ps[c]).im;
+                    if (t < p->trap_off_time)
+                        for (k = 0; k < 3; ++k) sens[4 + k] += p->trap[k] * r[k] * r[k] * im_mp;
+                    if (F > 0) {
+                        const double s = (F == 1) ? 1.0 : -1.0;
+                        double pb[3];
+                        for (k = 0; k < 3; ++k) {
+                            double e[3] = {0, 0, 0};
+                            cplx fp[5] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+                            e[k] = 1.0;
+                            spin_dot_apply(F, e, ps, fp);
+                            double im = 0;
+                            for (c = 0; c < C; ++c) im += cmul(cconj(mu[c]), fp[c]).im;
+                            pb[k] = s * im;
+                            sens[k] += pb[k];
+                        }
+                        sens[3] += (r[0] - p->center[0]) * pb[0] + (r[1] - p->center[1]) * pb[1] -
+                                   2.0 * (r[2] - p->center[2]) * pb[2];
+                    }
+                }
+}

@@ -23,7 +23,7 @@
  *     psi[2 * (c * ncell * V + ((z * ny + y) * nx + x) * V + v) + {0 re, 1 im}]
  * Values outside the box are zero (Dirichlet), as the reference's never-written halo.
  *
- * Parity status of every function is stated in DESIGN.md and in the tests.
+ * The tests state, per function, what pins it (tests/test_oracle.py, tests/test_tables.py).
  */
 #ifndef GPE_ORACLE_H
 #define GPE_ORACLE_H
@@ -146,6 +146,11 @@ int orc_gpblocks_integrate(const orc_lattice *L, int kappa, double g, const doub
                            double rplug, int zsize, double zstep, double mu, double block_scale, const double minp[3],
                            const double maxp[3], unsigned period, unsigned iterations, double *out_normsq,
                            double *out_error, double *out_dt, int *out_steps);
+
+/* Discrete adjoint of the 3-level leapfrog, one reverse sweep (see gpe_oracle.c):
+ * mu2 += coef G'(psi)^T mu1 and the seven parameter sensitivities at this sweep. */
+void orc_adjoint_sweep(const orc_lattice *L, const orc_params *p, double t, double coef, const double *psi,
+                       const double *mu1, double *mu2, double sens[7]);
 
 int orc_set_threads(int n);
 

@@ -203,6 +203,7 @@ SweepArgs sweep_args(gpe_ctx *c, int mode_coef_sweep, const void *cur, void *nex
     a.ph = c->ph;
     a.cur = cur;
     a.next = next;
+    a.prev = nullptr;
     a.fields = c->d_fields;
     a.sweep = mode_coef_sweep;
     a.quad = quad_active(c) ? 1 : 0;
@@ -276,6 +277,7 @@ int gpe_create(const gpe_config *cfg, gpe_ctx **out)
     ph.cell = cfg->block_scale * c->lat->block_width;
     for (int k = 0; k < 3; ++k) {
         ph.p0[k] = cfg->p0[k];
+        ph.trap[k] = cfg->trap[k];
         ph.trap2[k] = cfg->trap[k] * cfg->trap[k];
         ph.center[k] = cfg->center[k];
     }
@@ -635,6 +637,285 @@ int gpe_time(const gpe_ctx *c, double *t, long long *sweeps)
     if (!c) return GPE_EINVAL;
     if (t) *t = c->t0 + (double)c->sweeps * sweep_dt(c);
     if (sweeps) *sweeps = c->sweeps;
+    return GPE_OK;
+}
+
+int gpe_set_trap(gpe_ctx *c, const double trap[3])
+{
+    if (!c || !trap) return GPE_EINVAL;
+    for (int k = 0; k < 3; ++k) {
+        c->cfg.trap[k] = trap[k];
+        c->ph.trap[k] = trap[k];
+        c->ph.trap2[k] = trap[k] * trap[k];
+    }
+    return GPE_OK;
+}
+
+// Ramp sensitivities: the ramp value is linear in the segment values, so
+// d value(t) / d val[j] = value(t) evaluated with val = e_j.
+static double ramp_basis(const std::vector<double> &dur, const std::vector<int> &type, double tau, double t, int j)
+{
+    std::vector<double> e(dur.size(), 0.0);
+    e[j] = 1.0;
+    double out;
+    ramp_channel(e, dur, type, 1, tau, t, &out);
+    return out;
+}
+
+int gpe_adjoint(gpe_ctx *c, const gpe_adjoint_spec *spec, gpe_adjoint_result *res)
+{
+    if (!c) return GPE_EINVAL;
+    if (!spec || !res) return fail(c, GPE_EINVAL, "null adjoint spec or result");
+    if (c->cfg.scheme != GPE_SCHEME_LEAPFROG3) return fail(c, GPE_EINVAL, "adjoint needs the 3-level scheme");
+    if (c->cfg.alpha != 0.0) return fail(c, GPE_EINVAL, "adjoint needs alpha = 0");
+    if (c->ramp.active && (c->ramp.qscale_q != 0.0 || c->ramp.qscale_b != 0.0))
+        return fail(c, GPE_EINVAL, "adjoint does not differentiate a ramped quadratic field");
+    const int N = spec->nsweeps, K = spec->checkpoint_every;
+    if (N < 1 || K < 2) return fail(c, GPE_EINVAL, "need nsweeps >= 1 and checkpoint_every >= 2");
+    if (spec->objective < 0 || spec->objective > 2) return fail(c, GPE_EINVAL, "unknown objective");
+    if (spec->objective == GPE_OBJ_OVERLAP && !spec->target) return fail(c, GPE_EINVAL, "overlap needs a target");
+    if (c->sweeps != 0) return fail(c, GPE_ESTATE, "adjoint must start right after gpe_set_state");
+
+    const size_t lbytes = (size_t)c->level_elems * c->elem_bytes;
+    const int nck = (N + K - 1) / K;
+    std::vector<void *> bufs;
+    auto alloc = [&](void **p, size_t bytes) -> int {
+        if (hipMalloc(p, bytes) != hipSuccess) return GPE_ENOMEM;
+        bufs.push_back(*p);
+        return GPE_OK;
+    };
+    auto release = [&]() {
+        (void)hipStreamSynchronize(c->stream);
+        for (void *p : bufs) (void)hipFree(p);
+        bufs.clear();
+    };
+    std::vector<void *> ck(2 * nck, nullptr), seg(K, nullptr);
+    void *mu[2] = {nullptr, nullptr};
+    void *target = nullptr;
+    double *d_steps = nullptr, *d_part = nullptr;
+    int rc = GPE_OK;
+    for (int i = 0; i < 2 * nck && !rc; ++i) rc = alloc(&ck[i], lbytes);
+    for (int i = 2; i < K && !rc; ++i) rc = alloc(&seg[i], lbytes);  // seg[0], seg[1] alias checkpoints
+    for (int i = 0; i < 2 && !rc; ++i) rc = alloc(&mu[i], lbytes);
+    if (!rc && spec->objective == GPE_OBJ_OVERLAP) rc = alloc(&target, lbytes);
+    if (!rc) rc = alloc((void **)&d_steps, sizeof(double) * ADJ_COUNT * (size_t)N);
+    if (!rc) rc = alloc((void **)&d_part, sizeof(double) * ADJ_COUNT * (size_t)c->g.blocks);
+    if (rc) {
+        release();
+        return fail(c, rc, "adjoint buffers do not fit");
+    }
+    res->checkpoint_bytes = (double)lbytes * 2 * nck;
+    res->segment_bytes = (double)lbytes * (K - 2 + 2);
+    hipError_t e = hipSuccess;
+#define ADJ_CHECK(call, what)                 \
+    do {                                      \
+        e = (call);                           \
+        if (e != hipSuccess) {                \
+            release();                        \
+            return hip_fail(c, e, what);      \
+        }                                     \
+    } while (0)
+
+    const double sdt = sweep_dt(c);
+    rc = upload_fields(c, 0, N);
+    if (rc) {
+        release();
+        return rc;
+    }
+    // ---- forward pass, keeping (psi_jK, psi_jK+1)
+    HIPCALL(c, hipEventRecord(c->ev0, c->stream));
+    ADJ_CHECK(hipMemcpyAsync(ck[0], c->level[0], lbytes, hipMemcpyDeviceToDevice, c->stream), "checkpoint");
+    {
+        SweepArgs a = sweep_args(c, 0, c->level[0], c->level[1], sdt);
+        ADJ_CHECK(launch_sweep(c->fp64, c->cfg.ncomp, MODE_EULER, a, c->stream), "forward Euler");
+        std::swap(c->level[0], c->level[1]);
+    }
+    ADJ_CHECK(hipMemcpyAsync(ck[1], c->level[0], lbytes, hipMemcpyDeviceToDevice, c->stream), "checkpoint");
+    for (int k = 1; k < N; ++k) {  // produces psi_{k+1}
+        SweepArgs a = sweep_args(c, k, c->level[0], c->level[1], 2.0 * sdt);
+        ADJ_CHECK(launch_sweep(c->fp64, c->cfg.ncomp, MODE_UPDATE, a, c->stream), "forward sweep");
+        std::swap(c->level[0], c->level[1]);
+        const int m = k + 1;
+        if (m < N && m % K == 0)
+            ADJ_CHECK(hipMemcpyAsync(ck[2 * (m / K)], c->level[0], lbytes, hipMemcpyDeviceToDevice, c->stream), "ck");
+        if (m <= N && m % K == 1 && m / K < nck)
+            ADJ_CHECK(hipMemcpyAsync(ck[2 * (m / K) + 1], c->level[0], lbytes, hipMemcpyDeviceToDevice, c->stream),
+                      "ck");
+    }
+    c->sweeps = N;
+    c->level1_valid = true;
+    HIPCALL(c, hipEventRecord(c->ev1, c->stream));
+    HIPCALL(c, hipEventSynchronize(c->ev1));
+    float fms = 0.f;
+    HIPCALL(c, hipEventElapsedTime(&fms, c->ev0, c->ev1));
+    res->forward_ms = fms;
+
+    // ---- objective and mu_N
+    HIPCALL(c, hipEventRecord(c->ev0, c->stream));
+    ObjArgs oa;
+    oa.g = c->g;
+    oa.tb = tables(c);
+    oa.ph = c->ph;
+    oa.field = field_at(c, c->t0 + N * sdt);
+    oa.psi = c->level[0];
+    oa.target = target;
+    oa.grad = mu[1];
+    oa.scalars = c->d_scalars + 32;
+    oa.partials = d_part;
+    ADJ_CHECK(hipMemsetAsync(mu[0], 0, lbytes, c->stream), "clear mu");
+    double J = 0.0;
+    double explicit_mom[4] = {0, 0, 0, 0};  // energy: sum dv r_a^2 n, s sum dv (dx Fx + dy Fy - 2 dz Fz)
+    double explicit_mag[3] = {0, 0, 0};
+    if (spec->objective == GPE_OBJ_OVERLAP) {
+        const Geometry &g = c->g;
+        const size_t abi = (size_t)c->cfg.ncomp * g.nz * g.ny * g.nx * g.V * 2 * sizeof(double);
+        double *staging = nullptr;
+        ADJ_CHECK(hipMalloc((void **)&staging, abi), "staging");
+        bufs.push_back(staging);
+        ADJ_CHECK(hipMemcpyAsync(staging, spec->target, abi, hipMemcpyHostToDevice, c->stream), "target");
+        ADJ_CHECK(launch_pack(c->fp64, g, c->cfg.ncomp, staging, target, c->stream), "pack target");
+        oa.mode = OBJ_PARTIALS;
+        ADJ_CHECK(launch_objective(c->fp64, c->cfg.ncomp, oa, c->stream), "overlap");
+        ADJ_CHECK(launch_finalize(d_part, g.blocks, 2, -1, c->d_scalars + 32, c->stream), "overlap sum");
+        oa.mode = OBJ_OVERLAP;
+        ADJ_CHECK(launch_objective(c->fp64, c->cfg.ncomp, oa, c->stream), "overlap grad");
+        double ov[2];
+        HIPCALL(c, hipMemcpyAsync(ov, c->d_scalars + 32, sizeof(ov), hipMemcpyDeviceToHost, c->stream));
+        HIPCALL(c, hipStreamSynchronize(c->stream));
+        J = ov[0] * ov[0] + ov[1] * ov[1];
+    } else {
+        gpe_observables_t o;
+        int orc = gpe_observables(c, 0, &o);
+        if (orc) {
+            release();
+            return orc;
+        }
+        if (spec->objective == GPE_OBJ_ENERGY) {
+            J = o.energy;
+            for (int k = 0; k < 3; ++k) explicit_mag[k] = o.magnetization[k];
+            oa.mode = OBJ_EXPLICIT;
+            ADJ_CHECK(launch_objective(c->fp64, c->cfg.ncomp, oa, c->stream), "explicit terms");
+            ADJ_CHECK(launch_finalize(d_part, c->g.blocks, 4, -1, c->d_scalars + 40, c->stream), "explicit sums");
+            HIPCALL(c, hipMemcpyAsync(explicit_mom, c->d_scalars + 40, sizeof(explicit_mom), hipMemcpyDeviceToHost,
+                                      c->stream));
+            HIPCALL(c, hipStreamSynchronize(c->stream));
+            // H psi_N at the time of psi_N into mu[1], then scaled by 2 dv
+            int frc = upload_fields(c, N, 1);
+            if (frc) {
+                release();
+                return frc;
+            }
+            SweepArgs a = sweep_args(c, 0, c->level[0], mu[1], 0.0);
+            ADJ_CHECK(launch_sweep(c->fp64, c->cfg.ncomp, MODE_HPSI, a, c->stream), "H psi");
+            oa.mode = OBJ_ENERGY;
+        } else {
+            J = o.magnetization[2];
+            oa.mode = OBJ_MAGZ;
+        }
+        ADJ_CHECK(launch_objective(c->fp64, c->cfg.ncomp, oa, c->stream), "objective grad");
+        int frc = upload_fields(c, 0, N);
+        if (frc) {
+            release();
+            return frc;
+        }
+    }
+    res->objective = J;
+
+    // ---- backward pass over the segments, recomputing psi inside each
+    // A = mu_{k+2}, B = mu_{k+1}
+    void *A = mu[0], *B = mu[1];
+    for (int j = nck - 1; j >= 0; --j) {
+        const int klo = j * K, khi = std::min((j + 1) * K - 1, N - 1);
+        seg[0] = ck[2 * j];
+        seg[1] = ck[2 * j + 1];
+        for (int m = klo + 1; m < khi; ++m) {  // psi_{m+1} = psi_{m-1} - 2i dt H psi_m
+            SweepArgs a = sweep_args(c, m, seg[m - klo], seg[m + 1 - klo], 2.0 * sdt);
+            a.prev = seg[m - 1 - klo];
+            ADJ_CHECK(launch_sweep(c->fp64, c->cfg.ncomp, MODE_UPDATE, a, c->stream), "recompute");
+        }
+        for (int k = khi; k >= klo; --k) {
+            AdjArgs aa;
+            aa.g = c->g;
+            aa.tb = tables(c);
+            aa.ph = c->ph;
+            aa.psi = seg[k - klo];
+            aa.mu1 = B;
+            aa.mu2 = A;
+            aa.fields = c->d_fields;
+            aa.sweep = k;
+            aa.quad = quad_active(c);
+            aa.coef = 2.0 * sdt;
+            aa.partials = d_part;
+            ADJ_CHECK(launch_adjoint(c->fp64, c->cfg.ncomp, aa, c->stream), "adjoint sweep");
+            ADJ_CHECK(launch_finalize(d_part, c->g.blocks, ADJ_COUNT, -1, d_steps + (size_t)ADJ_COUNT * k, c->stream),
+                      "adjoint sums");
+            std::swap(A, B);
+        }
+    }
+    std::vector<double> steps((size_t)ADJ_COUNT * N);
+    HIPCALL(c, hipMemcpyAsync(steps.data(), d_steps, sizeof(double) * steps.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCALL(c, hipEventRecord(c->ev1, c->stream));
+    HIPCALL(c, hipEventSynchronize(c->ev1));
+    float bms = 0.f;
+    HIPCALL(c, hipEventElapsedTime(&bms, c->ev0, c->ev1));
+    res->backward_ms = bms;
+    release();
+
+    // ---- chain rule to the parameters (FP64 on the host)
+    for (int a = 0; a < 3; ++a) {
+        res->grad_trap[a] = 0.0;
+        res->grad_static_bb[a] = 0.0;
+    }
+    res->grad_static_bq = 0.0;
+    const Ramp &r = c->ramp;
+    if (res->grad_ramp_q)
+        for (size_t j = 0; j < r.q_dur.size(); ++j) res->grad_ramp_q[j] = 0.0;
+    if (res->grad_ramp_b)
+        for (size_t j = 0; j < 3 * r.b_dur.size(); ++j) res->grad_ramp_b[j] = 0.0;
+    for (int k = 0; k < N; ++k) {
+        const double *p = &steps[(size_t)ADJ_COUNT * k];
+        const double w = (k == 0) ? sdt : 2.0 * sdt;
+        for (int a = 0; a < 3; ++a) res->grad_trap[a] += w * p[ADJ_WX + a];
+        if (!r.active) {
+            for (int a = 0; a < 3; ++a) res->grad_static_bb[a] += w * p[ADJ_BX + a];
+            res->grad_static_bq += w * p[ADJ_BQ];
+        } else {
+            const double tr = (c->t0 + k * sdt) * r.time_scale;
+            if (res->grad_ramp_q)
+                for (size_t j = 0; j < r.q_dur.size(); ++j)
+                    res->grad_ramp_q[j] += w * r.scale_q * ramp_basis(r.q_dur, r.q_type, r.tau, tr, (int)j) * p[ADJ_BQ];
+            if (res->grad_ramp_b)
+                for (size_t j = 0; j < r.b_dur.size(); ++j) {
+                    const double phi = ramp_basis(r.b_dur, r.b_type, r.tau, tr, (int)j);
+                    if (phi == 0.0) continue;
+                    for (int a = 0; a < 3; ++a) res->grad_ramp_b[3 * j + a] += w * r.scale_b * phi * p[ADJ_BX + a];
+                }
+        }
+    }
+    if (spec->objective == GPE_OBJ_ENERGY) {
+        // explicit dependence of E[psi_N] on the parameters at t_N
+        const double tN = c->t0 + N * sdt;
+        if (tN < c->ph.trap_off_time)
+            for (int a = 0; a < 3; ++a) res->grad_trap[a] += c->ph.trap[a] * explicit_mom[a];
+        if (c->cfg.ncomp > 1) {
+            const double s = c->cfg.ncomp == 3 ? 1.0 : -1.0;
+            if (!r.active) {
+                for (int a = 0; a < 3; ++a) res->grad_static_bb[a] += s * explicit_mag[a];
+                res->grad_static_bq += explicit_mom[3];
+            } else {
+                const double tr = tN * r.time_scale;
+                if (res->grad_ramp_q)
+                    for (size_t j = 0; j < r.q_dur.size(); ++j)
+                        res->grad_ramp_q[j] += r.scale_q * ramp_basis(r.q_dur, r.q_type, r.tau, tr, (int)j) * explicit_mom[3];
+                if (res->grad_ramp_b)
+                    for (size_t j = 0; j < r.b_dur.size(); ++j) {
+                        const double phi = ramp_basis(r.b_dur, r.b_type, r.tau, tr, (int)j);
+                        for (int a = 0; a < 3; ++a) res->grad_ramp_b[3 * j + a] += r.scale_b * phi * s * explicit_mag[a];
+                    }
+            }
+        }
+    }
+#undef ADJ_CHECK
     return GPE_OK;
 }
 

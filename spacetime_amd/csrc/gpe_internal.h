@@ -40,6 +40,7 @@ struct Tables {
 struct Physics {
     double cell;           // s * block_width
     double p0[3];
+    double trap[3];        // trap frequencies
     double trap2[3];       // squared trap frequencies
     double trap_offset, trap_off_time;
     double c0, c1, c2, alpha, quad_sign, q;
@@ -62,6 +63,7 @@ struct SweepArgs {
     Physics ph;
     const void *cur;          // level read by the stencil (psi_k)
     void *next;               // level written (psi_{k-1} -> psi_{k+1})
+    const void *prev;         // update: psi_{k-1} read from here instead of next (may be null)
     const FieldRec *fields;   // field of sweep k at fields[k]
     int sweep;                // index into fields
     int quad, loss;           // quadratic Zeeman / three-body loss active
@@ -82,6 +84,41 @@ struct ObsArgs {
     int quad;
     double *partials;         // OBS_COUNT per workgroup
 };
+
+// Adjoint sweep: mu2 <- mu2 + coef G'(psi)^T mu1, plus per-workgroup parameter sensitivities.
+enum { ADJ_BX = 0, ADJ_BY, ADJ_BZ, ADJ_BQ, ADJ_WX, ADJ_WY, ADJ_WZ, ADJ_COUNT };
+
+struct AdjArgs {
+    Geometry g;
+    Tables tb;
+    Physics ph;
+    const void *psi;          // psi_k
+    const void *mu1;          // mu_{k+1} (stencil operand)
+    void *mu2;                // mu_{k+2} in, mu_k out
+    const FieldRec *fields;
+    int sweep;
+    int quad;
+    double coef;
+    double *partials;         // ADJ_COUNT per workgroup
+};
+
+enum { OBJ_PARTIALS = 0, OBJ_OVERLAP = 1, OBJ_ENERGY = 2, OBJ_MAGZ = 3, OBJ_EXPLICIT = 4 };
+
+struct ObjArgs {
+    Geometry g;
+    Tables tb;
+    Physics ph;
+    FieldRec field;           // field at the time of psi (OBJ_EXPLICIT)
+    int mode;
+    const void *psi;
+    const void *target;
+    void *grad;
+    const double *scalars;    // OBJ_OVERLAP: (Re, Im) of <target|psi>
+    double *partials;         // OBJ_PARTIALS: 2 per workgroup; OBJ_EXPLICIT: 4 per workgroup
+};
+
+hipError_t launch_adjoint(int fp64, int ncomp, const AdjArgs &a, hipStream_t s);
+hipError_t launch_objective(int fp64, int ncomp, const ObjArgs &a, hipStream_t s);
 
 // launchers (gpe_kernels.hip)
 hipError_t launch_sweep(int fp64, int ncomp, int mode, const SweepArgs &a, hipStream_t s);

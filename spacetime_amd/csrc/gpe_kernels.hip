@@ -13,245 +13,11 @@
 // Workgroups are remapped so that each XCD walks a contiguous slab of the traversal
 // order (y-tiles, then z, then rows of the tile): the z +- 1 and y +- 1 rows a wave needs
 // were fetched into the same XCD's L2 by its neighbours shortly before.
-#include "gpe_internal.h"
-
-#include <type_traits>
+#include "gpe_device.h"
 
 namespace gpe {
 
-template <typename R>
-using vec2_t = typename std::conditional<std::is_same<R, float>::value, float2, double2>::type;
-
-__device__ __forceinline__ int logical_block(int b, int nb)
-{
-    // hardware dispatch is round-robin over the 8 XCDs: give XCD k a contiguous range
-    const int q = nb >> 3, r = nb & 7, k = b & 7;
-    return k * q + (k < r ? k : r) + (b >> 3);
-}
-
-struct WaveCoord {
-    int x, v, y, z;
-    bool valid;
-};
-
-__device__ __forceinline__ WaveCoord decode(const Geometry &g)
-{
-    WaveCoord wc;
-    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const long long w = (long long)logical_block(blockIdx.x, gridDim.x) * 4 + wid;
-    long long r = w;
-    const int xseg = (int)(r % g.nseg);
-    r /= g.nseg;
-    wc.v = (int)(r % g.V);
-    r /= g.V;
-    const int yin = (int)(r % g.ty);
-    r /= g.ty;
-    wc.z = (int)(r % g.nz);
-    const int tile = (int)(r / g.nz);
-    wc.y = tile * g.ty + yin;
-    wc.valid = (w < g.waves) && (wc.y < g.ny);
-    wc.x = xseg * 64 + (int)(threadIdx.x & 63);
-    return wc;
-}
-
-__device__ __forceinline__ long long line_index(const Geometry &g, int c, int z, int y, int v)
-{
-    return ((long long)((c * g.nz + z) * g.ny + y) * g.V + v) * g.lx;
-}
-
-// ------------------------------------------------------------------ local Hamiltonian --
-
-// (W . f) psi for spin-1 (m = +1, 0, -1): Spin1GpeBCC.cu:890-895 expanded form.
-template <typename R>
-__device__ __forceinline__ void spin1_dot(R wx, R wy, R wz, const R *pr, const R *pi, R *orr, R *oi)
-{
-    const R s = (R)0.70710678118654752440;
-    const R pre = s * wx, pim = s * wy;  // W+ / sqrt2 = (pre, pim); W- / sqrt2 = (pre, -pim)
-    // m = +1: wz psi1 + W-/sqrt2 psi0
-    orr[0] = wz * pr[0] + pre * pr[1] + pim * pi[1];
-    oi[0] = wz * pi[0] + pre * pi[1] - pim * pr[1];
-    // m = 0: W+/sqrt2 psi1 + W-/sqrt2 psi-1
-    orr[1] = pre * pr[0] - pim * pi[0] + pre * pr[2] + pim * pi[2];
-    oi[1] = pre * pi[0] + pim * pr[0] + pre * pi[2] - pim * pr[2];
-    // m = -1: W+/sqrt2 psi0 - wz psi-1
-    orr[2] = pre * pr[1] - pim * pi[1] - wz * pr[2];
-    oi[2] = pre * pi[1] + pim * pr[1] - wz * pi[2];
-}
-
-// (W . f) psi for spin-2 (m = 2..-2): ladder factors 2, sqrt6, sqrt6, 2 (Spin2GpeBCC.cu:954-970).
-template <typename R>
-__device__ __forceinline__ void spin2_dot(R wx, R wy, R wz, const R *pr, const R *pi, R *orr, R *oi)
-{
-    const R s6 = (R)2.44948974278317809820;
-    const R hr = (R)0.5 * wx, hi = (R)0.5 * wy;  // W+/2 = (hr, hi), W-/2 = (hr, -hi)
-    // lower(m) = W-/2 * a_m * psi_{m-1}; upper(m) = W+/2 * b_m * psi_{m+1}
-#define GPE_LOWER(idx, a, k) \
-    (a) * (hr * pr[k] + hi * pi[k]), (a) * (hr * pi[k] - hi * pr[k])
-#define GPE_UPPER(idx, b, k) \
-    (b) * (hr * pr[k] - hi * pi[k]), (b) * (hr * pi[k] + hi * pr[k])
-    R lr, li, ur, ui;
-    // m = 2
-    {
-        R a0 = 2 * (hr * pr[1] + hi * pi[1]), a1 = 2 * (hr * pi[1] - hi * pr[1]);
-        orr[0] = 2 * wz * pr[0] + a0;
-        oi[0] = 2 * wz * pi[0] + a1;
-    }
-    // m = 1
-    lr = s6 * (hr * pr[2] + hi * pi[2]); li = s6 * (hr * pi[2] - hi * pr[2]);
-    ur = 2 * (hr * pr[0] - hi * pi[0]); ui = 2 * (hr * pi[0] + hi * pr[0]);
-    orr[1] = wz * pr[1] + lr + ur;
-    oi[1] = wz * pi[1] + li + ui;
-    // m = 0
-    lr = s6 * (hr * pr[3] + hi * pi[3]); li = s6 * (hr * pi[3] - hi * pr[3]);
-    ur = s6 * (hr * pr[1] - hi * pi[1]); ui = s6 * (hr * pi[1] + hi * pr[1]);
-    orr[2] = lr + ur;
-    oi[2] = li + ui;
-    // m = -1
-    lr = 2 * (hr * pr[4] + hi * pi[4]); li = 2 * (hr * pi[4] - hi * pr[4]);
-    ur = s6 * (hr * pr[2] - hi * pi[2]); ui = s6 * (hr * pi[2] + hi * pr[2]);
-    orr[3] = -wz * pr[3] + lr + ur;
-    oi[3] = -wz * pi[3] + li + ui;
-    // m = -2
-    ur = 2 * (hr * pr[3] - hi * pi[3]); ui = 2 * (hr * pi[3] + hi * pr[3]);
-    orr[4] = -2 * wz * pr[4] + ur;
-    oi[4] = -2 * wz * pi[4] + ui;
-#undef GPE_LOWER
-#undef GPE_UPPER
-}
-
-template <typename R, int C>
-__device__ __forceinline__ void spin_dot(R wx, R wy, R wz, const R *pr, const R *pi, R *orr, R *oi)
-{
-    if constexpr (C == 3)
-        spin1_dot<R>(wx, wy, wz, pr, pi, orr, oi);
-    else
-        spin2_dot<R>(wx, wy, wz, pr, pi, orr, oi);
-}
-
-struct LocalCoef {
-    double vtrap;
-    double Bx, By, Bz;
-    double Qx, Qy, Qz;
-};
-
-// hr/hi += local part of H psi at one unknown.
-template <typename R, int C>
-__device__ __forceinline__ void add_local_h(const R *pr, const R *pi, R vtrap, R Bx, R By, R Bz, R Qx, R Qy, R Qz,
-                                            R c0, R c1, R c2, R alpha, R quad_sign, R q, int quad, int loss,
-                                            R *hr, R *hi)
-{
-    R n = 0;
-#pragma unroll
-    for (int c = 0; c < C; ++c) n += pr[c] * pr[c] + pi[c] * pi[c];
-    const R d = vtrap + c0 * n;
-    const R l = loss ? alpha * n * n : (R)0;
-    // (d - i l) psi
-#pragma unroll
-    for (int c = 0; c < C; ++c) {
-        hr[c] += d * pr[c] + l * pi[c];
-        hi[c] += d * pi[c] - l * pr[c];
-    }
-    if constexpr (C > 1) {
-        R Fx, Fy, Fz, wx, wy, wz;
-        if constexpr (C == 3) {
-            // F+ = sqrt2 (conj(psi1) psi0 + conj(psi0) psi-1), Fz = |psi1|^2 - |psi-1|^2
-            const R s2 = (R)1.41421356237309504880;
-            Fx = s2 * (pr[0] * pr[1] + pi[0] * pi[1] + pr[1] * pr[2] + pi[1] * pi[2]);
-            Fy = s2 * (pr[0] * pi[1] - pi[0] * pr[1] + pr[1] * pi[2] - pi[1] * pr[2]);
-            Fz = pr[0] * pr[0] + pi[0] * pi[0] - pr[2] * pr[2] - pi[2] * pi[2];
-            wx = Bx + c1 * Fx;
-            wy = By + c1 * Fy;
-            wz = Bz + c1 * Fz;
-        } else {
-            const R s6 = (R)2.44948974278317809820;
-            // F+ = 2 c(psi2) psi1 + s6 c(psi1) psi0 + s6 c(psi0) psi-1 + 2 c(psi-1) psi-2
-            Fx = 2 * (pr[0] * pr[1] + pi[0] * pi[1]) + s6 * (pr[1] * pr[2] + pi[1] * pi[2]) +
-                 s6 * (pr[2] * pr[3] + pi[2] * pi[3]) + 2 * (pr[3] * pr[4] + pi[3] * pi[4]);
-            Fy = 2 * (pr[0] * pi[1] - pi[0] * pr[1]) + s6 * (pr[1] * pi[2] - pi[1] * pr[2]) +
-                 s6 * (pr[2] * pi[3] - pi[2] * pr[3]) + 2 * (pr[3] * pi[4] - pi[3] * pr[4]);
-            Fz = 2 * (pr[0] * pr[0] + pi[0] * pi[0]) + pr[1] * pr[1] + pi[1] * pi[1] - pr[3] * pr[3] -
-                 pi[3] * pi[3] - 2 * (pr[4] * pr[4] + pi[4] * pi[4]);
-            // spin-2 linear Zeeman enters with -B (Spin2GpeBCC.cu:942-957)
-            wx = c1 * Fx - Bx;
-            wy = c1 * Fy - By;
-            wz = c1 * Fz - Bz;
-        }
-        R tr[C], ti[C];
-        spin_dot<R, C>(wx, wy, wz, pr, pi, tr, ti);
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-            hr[c] += tr[c];
-            hi[c] += ti[c];
-        }
-        if constexpr (C == 5) {
-            // singlet: H_m += (c2/5) Theta (-1)^m conj(psi_-m), Theta = 2 p2 p-2 - 2 p1 p-1 + p0^2
-            const R thr = 2 * (pr[0] * pr[4] - pi[0] * pi[4]) - 2 * (pr[1] * pr[3] - pi[1] * pi[3]) +
-                          (pr[2] * pr[2] - pi[2] * pi[2]);
-            const R thi = 2 * (pr[0] * pi[4] + pi[0] * pr[4]) - 2 * (pr[1] * pi[3] + pi[1] * pr[3]) +
-                          2 * pr[2] * pi[2];
-            const R k = c2 * (R)0.2;
-#pragma unroll
-            for (int c = 0; c < C; ++c) {
-                const R sg = (c & 1) ? -k : k;
-                const R cr = pr[C - 1 - c], ci = -pi[C - 1 - c];
-                hr[c] += sg * (thr * cr - thi * ci);
-                hi[c] += sg * (thr * ci + thi * cr);
-            }
-        }
-        if (quad) {
-            R ur[C], ui[C];
-            spin_dot<R, C>(Qx, Qy, Qz, pr, pi, tr, ti);
-            spin_dot<R, C>(Qx, Qy, Qz, tr, ti, ur, ui);
-#pragma unroll
-            for (int c = 0; c < C; ++c) {
-                hr[c] += quad_sign * ur[c];
-                hi[c] += quad_sign * ui[c];
-            }
-        }
-        constexpr int F = (C - 1) / 2;
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-            const R mm = (R)((F - c) * (F - c));
-            hr[c] += q * mm * pr[c];
-            hi[c] += q * mm * pi[c];
-        }
-    }
-}
-
 // ------------------------------------------------------------------------- the sweep --
-
-template <typename R>
-__device__ __forceinline__ const R *table_w(const Tables &t);
-template <>
-__device__ __forceinline__ const float *table_w<float>(const Tables &t) { return t.w32; }
-template <>
-__device__ __forceinline__ const double *table_w<double>(const Tables &t) { return t.w64; }
-template <typename R>
-__device__ __forceinline__ const R *table_wsum(const Tables &t);
-template <>
-__device__ __forceinline__ const float *table_wsum<float>(const Tables &t) { return t.wsum32; }
-template <>
-__device__ __forceinline__ const double *table_wsum<double>(const Tables &t) { return t.wsum64; }
-template <typename R>
-__device__ __forceinline__ const R *table_pos(const Tables &t);
-template <>
-__device__ __forceinline__ const float *table_pos<float>(const Tables &t) { return t.pos32; }
-template <>
-__device__ __forceinline__ const double *table_pos<double>(const Tables &t) { return t.pos64; }
-
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-__device__ __forceinline__ double wave_max(double v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-    return v;
-}
 
 template <typename R, int C, int DEG, int MODE>
 __global__ __launch_bounds__(256) void k_sweep(SweepArgs a)
@@ -267,19 +33,34 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a)
         const int xc = act ? x : g.nx - 1;
         const V2 *__restrict__ cur = (const V2 *)a.cur;
         V2 *__restrict__ nxt = (V2 *)a.next;
+        const V2 *__restrict__ prv = (const V2 *)(a.prev ? a.prev : a.next);
+        const long long cstride = (long long)g.nz * g.ny * g.V * g.lx;  // elements per component
+        const long long own_line = line_index(g, 0, z, y, v);
 
-        R pr[C], pi[C], ar[C], ai[C];
+        // issue every load of the wave before any use: own values, psi_{k-1}, and the
+        // DEG neighbour lines (wave-uniform line bases, per-lane 32-bit offsets)
+        R pr[C], pi[C], qr[C], qi[C], ar[C], ai[C];
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-            const V2 o = cur[line_index(g, c, z, y, v) + xc];
+            const V2 o = cur[own_line + c * cstride + xc];
             pr[c] = o.x;
             pi[c] = o.y;
             ar[c] = 0;
             ai[c] = 0;
         }
-        // stencil: sum_f w_f psi_t(f), out-of-box neighbours have weight 0
+        if constexpr (MODE == MODE_UPDATE) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const V2 o = nt_load<R>(&prv[own_line + c * cstride + xc]);
+                qr[c] = o.x;
+                qi[c] = o.y;
+            }
+        }
         const int *face = a.tb.face + v * DEG;
         const R *wt = table_w<R>(a.tb) + v * DEG;
+        R wf[DEG];
+        const V2 *nbase[DEG];
+        int noff[DEG];
 #pragma unroll
         for (int f = 0; f < DEG; ++f) {
             const int pk = face[f];
@@ -292,15 +73,23 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a)
             }
             const int xn = x + dx;
             const bool xok = (unsigned)xn < (unsigned)g.nx;
-            const int xnc = xn < 0 ? 0 : (xn >= g.nx ? g.nx - 1 : xn);
-            const R wf = (inside && xok) ? wt[f] : (R)0;
+            noff[f] = xn < 0 ? 0 : (xn >= g.nx ? g.nx - 1 : xn);
+            wf[f] = (inside && xok) ? wt[f] : (R)0;
+            nbase[f] = cur + line_index(g, 0, zn, yn, t);
+        }
+        V2 nb[DEG][C];
+#pragma unroll
+        for (int f = 0; f < DEG; ++f)
+#pragma unroll
+            for (int c = 0; c < C; ++c) nb[f][c] = nbase[f][c * cstride + noff[f]];
+        // stencil: sum_f w_f psi_t(f); out-of-box neighbours carry weight 0
+#pragma unroll
+        for (int f = 0; f < DEG; ++f)
 #pragma unroll
             for (int c = 0; c < C; ++c) {
-                const V2 nb = cur[line_index(g, c, zn, yn, t) + xnc];
-                ar[c] += wf * nb.x;
-                ai[c] += wf * nb.y;
+                ar[c] += wf[f] * nb[f][c].x;
+                ai[c] += wf[f] * nb[f][c].y;
             }
-        }
         const R wsum = table_wsum<R>(a.tb)[v];
         R hr[C], hi[C];
         R sc = (R)1;
@@ -342,12 +131,11 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a)
         const R coef = (R)a.coef;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-            const long long idx = line_index(g, c, z, y, v) + xc;
+            const long long idx = own_line + c * cstride + xc;
             V2 o;
             if constexpr (MODE == MODE_UPDATE) {
-                o = nxt[idx];
-                o.x += coef * hi[c];
-                o.y -= coef * hr[c];
+                o.x = qr[c] + coef * hi[c];
+                o.y = qi[c] - coef * hr[c];
             } else if constexpr (MODE == MODE_EULER) {
                 o.x = pr[c] + coef * hi[c];
                 o.y = pi[c] - coef * hr[c];
@@ -359,7 +147,7 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a)
                 o.x = hr[c];
                 o.y = hi[c];
             }
-            if (act) nxt[idx] = o;
+            if (act) nt_store<R>(o, &nxt[idx]);
         }
         if constexpr (MODE == MODE_ITP) part *= a.tb.dv[v];
     }
@@ -509,22 +297,34 @@ __global__ __launch_bounds__(256) void k_observables(ObsArgs a)
 
 // --------------------------------------------------------------------- small kernels --
 
-__global__ __launch_bounds__(256) void k_finalize(const double *partials, int n, int width, int maxcol, double *out)
+__global__ __launch_bounds__(1024) void k_finalize(const double *partials, int n, int width, int maxcol, double *out)
 {
-    __shared__ double red[256];
+    // deterministic: fixed per-thread strides and a fixed tree
+    __shared__ double red[1024];
     for (int col = 0; col < width; ++col) {
         const bool is_max = col == maxcol;
-        double acc = is_max ? 0.0 : 0.0;
-        for (int i = threadIdx.x; i < n; i += 256) {
-            const double v = partials[(long long)i * width + col];
-            acc = is_max ? fmax(acc, v) : acc + v;
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        int i = threadIdx.x;
+        for (; i + 7 * 1024 < n; i += 8 * 1024) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const double v = partials[(long long)(i + u * 1024) * width + col];
+                acc[u] = is_max ? fmax(acc[u], v) : acc[u] + v;
+            }
         }
-        red[threadIdx.x] = acc;
+        for (int u = 0; i < n; i += 1024, ++u) {
+            const double v = partials[(long long)i * width + col];
+            acc[u & 7] = is_max ? fmax(acc[u & 7], v) : acc[u & 7] + v;
+        }
+        double s = acc[0];
+#pragma unroll
+        for (int u = 1; u < 8; ++u) s = is_max ? fmax(s, acc[u]) : s + acc[u];
+        red[threadIdx.x] = s;
         __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s)
-                red[threadIdx.x] = is_max ? fmax(red[threadIdx.x], red[threadIdx.x + s])
-                                          : red[threadIdx.x] + red[threadIdx.x + s];
+        for (int st = 512; st > 0; st >>= 1) {
+            if ((int)threadIdx.x < st)
+                red[threadIdx.x] = is_max ? fmax(red[threadIdx.x], red[threadIdx.x + st])
+                                          : red[threadIdx.x] + red[threadIdx.x + st];
             __syncthreads();
         }
         if (threadIdx.x == 0) out[col] = red[0];
@@ -668,7 +468,7 @@ hipError_t launch_observables(int fp64, int ncomp, const ObsArgs &a, hipStream_t
 
 hipError_t launch_finalize(const double *partials, int n, int width, int maxcol, double *out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, partials, n, width, maxcol, out);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, partials, n, width, maxcol, out);
     return hipGetLastError();
 }
 

@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgpe.so")
 FP32, FP64 = 0, 1
 LEAPFROG3, STAGGERED = 0, 1
 START_EULER, START_PHASE, START_COPY = 0, 1, 2
+OBJ_OVERLAP, OBJ_ENERGY, OBJ_MAGNETIZATION_Z = 0, 1, 2
 ERRORS = {-22: "EINVAL", -12: "ENOMEM", -5: "EDEVICE", -71: "ESTATE", -34: "ERANGE"}
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -55,6 +56,20 @@ class _Obs(ctypes.Structure):
         ("norm", ctypes.c_double), ("energy", ctypes.c_double), ("expect_h", ctypes.c_double),
         ("magnetization", ctypes.c_double * 3), ("max_density", ctypes.c_double), ("time", ctypes.c_double),
         ("finite", ctypes.c_int),
+    ]
+
+
+class _AdjSpec(ctypes.Structure):
+    _fields_ = [("nsweeps", ctypes.c_int), ("checkpoint_every", ctypes.c_int), ("objective", ctypes.c_int),
+                ("target", ctypes.c_void_p)]
+
+
+class _AdjRes(ctypes.Structure):
+    _fields_ = [
+        ("objective", ctypes.c_double), ("grad_trap", ctypes.c_double * 3), ("grad_static_bq", ctypes.c_double),
+        ("grad_static_bb", ctypes.c_double * 3), ("grad_ramp_q", _dp), ("grad_ramp_b", _dp),
+        ("forward_ms", ctypes.c_double), ("backward_ms", ctypes.c_double),
+        ("checkpoint_bytes", ctypes.c_double), ("segment_bytes", ctypes.c_double),
     ]
 
 
@@ -93,6 +108,8 @@ def load_library(path=LIB_PATH):
     lib.gpe_sync.argtypes = [vp]
     lib.gpe_time.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_longlong)]
     lib.gpe_last_step_timing.argtypes = [vp, _dp, _dp]
+    lib.gpe_adjoint.argtypes = [vp, ctypes.POINTER(_AdjSpec), ctypes.POINTER(_AdjRes)]
+    lib.gpe_set_trap.argtypes = [vp, _dp]
     _LIB = lib
     return lib
 
@@ -175,6 +192,7 @@ class Stepper:
             raise GpeError("gpe_create failed: %s (%d)" % (ERRORS.get(rc, "?"), rc))
         self._h = h
         self._keep = None
+        self._ramp_sizes = (0, 0)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -239,9 +257,39 @@ class Stepper:
         r.tau, r.time_scale = tau, time_scale
         r.scale_q, r.scale_b, r.qscale_q, r.qscale_b = scale_q, scale_b, qscale_q, qscale_b
         self._check(self.lib.gpe_set_ramp(self._h, ctypes.byref(r)), "gpe_set_ramp")
+        self._ramp_sizes = (len(q_segments), len(b_segments))
 
     def clear_ramp(self):
         self._check(self.lib.gpe_set_ramp(self._h, None), "gpe_set_ramp")
+        self._ramp_sizes = (0, 0)
+
+    def set_trap(self, trap):
+        w = (ctypes.c_double * 3)(*trap)
+        self._check(self.lib.gpe_set_trap(self._h, w), "gpe_set_trap")
+
+    def adjoint(self, nsweeps, checkpoint_every, objective=OBJ_OVERLAP, target=None):
+        """Objective and its gradients by the discrete adjoint (gpe_adjoint).  Returns a dict
+        with J, grad_trap (3), grad_static_bq, grad_static_bb (3), grad_ramp_q (nq),
+        grad_ramp_b (nb, 3), timings and memory."""
+        spec = _AdjSpec()
+        spec.nsweeps, spec.checkpoint_every, spec.objective = int(nsweeps), int(checkpoint_every), objective
+        tgt = None
+        if target is not None:
+            tgt = np.ascontiguousarray(target, dtype=np.complex128)
+            if tgt.shape != self.config.shape:
+                raise ValueError("target shape %s != %s" % (tgt.shape, self.config.shape))
+            spec.target = tgt.ctypes.data
+        nq, nb = self._ramp_sizes
+        gq = np.zeros(max(nq, 1))
+        gb = np.zeros((max(nb, 1), 3))
+        res = _AdjRes()
+        res.grad_ramp_q = gq.ctypes.data_as(_dp)
+        res.grad_ramp_b = gb.ctypes.data_as(_dp)
+        self._check(self.lib.gpe_adjoint(self._h, ctypes.byref(spec), ctypes.byref(res)), "gpe_adjoint")
+        return dict(J=res.objective, grad_trap=np.array(res.grad_trap[:]), grad_static_bq=res.grad_static_bq,
+                    grad_static_bb=np.array(res.grad_static_bb[:]), grad_ramp_q=gq[:nq], grad_ramp_b=gb[:nb],
+                    forward_ms=res.forward_ms, backward_ms=res.backward_ms,
+                    checkpoint_bytes=res.checkpoint_bytes, segment_bytes=res.segment_bytes)
 
     # -- time stepping -------------------------------------------------------------------------
     def start(self, mode=START_EULER, mu=0.0):

@@ -88,6 +88,8 @@ class Oracle:
         L.orc_vortex_function.restype = ctypes.c_double
         L.orc_mt_seed.argtypes = [ctypes.POINTER(ctypes.c_uint), ctypes.c_uint]
         L.orc_mt_uniform_circle.argtypes = [ctypes.POINTER(ctypes.c_uint), _dp]
+        L.orc_ramp_apply.argtypes = [RP, ctypes.c_double, P]
+        L.orc_adjoint_sweep.argtypes = [LP, P, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_set_threads.restype = ctypes.c_int
         L.orc_gpblocks_integrate.argtypes = [LP, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_int, ctypes.c_double,
@@ -271,6 +273,83 @@ class Oracle:
             steps.ctypes.data_as(_ip))
         assert rc == 0
         return ns, er, float(dt[0]), int(steps[0])
+
+    def adjoint(self, L, p, lat, dt, nsweeps, psi0, objective, target=None, ramp=None, ramp_bases=None):
+        """FP64 oracle gradient by the discrete adjoint, storing every forward state.
+
+        objective: 0 overlap |<target|psi_N>|^2, 1 GP energy at t_N, 2 magnetisation M_z.
+        ramp_bases: optional (phi_q(t) -> [nq], phi_b(t) -> [nb]) callables giving the ramp
+        sensitivities; with a ramp, p's field is taken from it at every sweep.
+        Returns dict(J, grad_trap, grad_bias (3), grad_bq, grad_ramp_q, grad_ramp_b (nb,3), sens)."""
+        import copy
+        N = nsweeps
+        dv = p.block_scale ** 3 * lat.vol
+        dvb = dv[None, None, None, None, :]
+
+        def params_at(t):
+            q = copy.copy(p)
+            if ramp is not None:
+                self.lib.orc_ramp_apply(ctypes.byref(ramp), t, ctypes.byref(q))
+            return q
+
+        states = [psi0.copy()]
+        p0 = params_at(0.0)
+        states.append(self.forward_euler(L, p0, psi0, dt))
+        for k in range(1, N):
+            nxt = states[k - 1].copy()
+            self.update(L, params_at(k * dt), nxt, states[k], 2 * dt, t=k * dt)
+            states.append(nxt)
+        psiN = states[N]
+        pN = params_at(N * dt)
+        F = (p.ncomp - 1) // 2
+        if objective == 0:
+            c = np.sum(dvb * np.conj(target) * psiN)
+            J = abs(c) ** 2
+            g = 2 * c * dvb * target
+        elif objective == 1:
+            J = self.energy(L, pN, psiN, t=N * dt)
+            g = 2 * dvb * self.apply_h(L, pN, psiN, t=N * dt)
+        else:
+            J = self.magnetization(L, pN, psiN)[2]
+            m = np.arange(F, -F - 1, -1, dtype=float)[:, None, None, None, None]
+            g = 2 * dvb * m * psiN
+        g = np.ascontiguousarray(g.astype(np.complex128))
+        A = np.zeros_like(psiN)
+        B = g
+        sens = np.zeros((N, 7))
+        s7 = np.zeros(7)
+        for k in range(N - 1, -1, -1):
+            self.lib.orc_adjoint_sweep(ctypes.byref(L), ctypes.byref(params_at(k * dt)), k * dt, 2 * dt,
+                                       self._c(states[k]), self._c(B), self._c(A), s7.ctypes.data_as(_dp))
+            sens[k] = s7
+            A, B = B, A
+        w = np.full(N, 2 * dt)
+        w[0] = dt
+        res = dict(J=J, sens=sens, grad_trap=(w[:, None] * sens[:, 4:7]).sum(0),
+                   grad_bias=(w[:, None] * sens[:, 0:3]).sum(0), grad_bq=float((w * sens[:, 3]).sum()))
+        if objective == 1:
+            # explicit dependence of E on the parameters at t_N
+            pos = self.positions(L, pN, lat.V)
+            n = np.sum(np.abs(psiN) ** 2, axis=0)
+            mom = [float(np.sum(dvb[0] * pos[..., a] ** 2 * n)) for a in range(3)]
+            res["grad_trap"] = res["grad_trap"] + np.array([pN.trap[a] * mom[a] for a in range(3)])
+            if F > 0:
+                s = 1.0 if F == 1 else -1.0
+                Mv = self.magnetization(L, pN, psiN)
+                res["grad_bias"] = res["grad_bias"] + s * Mv
+                res["explicit_mag"] = s * Mv
+        if ramp_bases is not None:
+            phq, phb = ramp_bases
+            gq = 0.0
+            gb = 0.0
+            for k in range(N):
+                gq = gq + w[k] * phq(k * dt) * sens[k, 3]
+                gb = gb + w[k] * np.outer(phb(k * dt), sens[k, 0:3])
+            if objective == 1 and F > 0:
+                gb = gb + np.outer(phb(N * dt), res["explicit_mag"])
+            res["grad_ramp_q"] = gq
+            res["grad_ramp_b"] = gb
+        return res
 
     def set_threads(self, n):
         return self.lib.orc_set_threads(n)
