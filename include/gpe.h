@@ -159,6 +159,28 @@ int gpe_time(const gpe_ctx *ctx, double *t, long long *sweeps);
  * sweep (3 touches x ncomp x 2 x sizeof(real) per unknown). */
 int gpe_last_step_timing(const gpe_ctx *ctx, double *ms_per_sweep, double *bytes_per_sweep);
 
+/* ---------------------------------------------------------------- initial states ---- */
+/* Stationary vortex state f(r, z) e^{i kappa phi} of the scalar GPE in the trap
+ * 0.5 (r^2 + ratio^2 z^2), by the reference's axisymmetric relaxation
+ * (VortexState::setRange + iterateSolution, GrossPitaevskii/VortexState.cpp:42-196).
+ * The profile lives on rsize x zsize points (r-major) from rmin with steps rstep, zstep. */
+typedef struct {
+    int kappa;
+    double g;
+    double rmin, rmax, zmax, rstep, zstep;
+    double ratio;
+    int maxiters;
+    double eps;
+} gpe_vortex_params;
+
+/* Number of profile points (rsize * zsize) the parameters give, or GPE_EINVAL. */
+int gpe_vortex_size(const gpe_vortex_params *vp, int *rsize, int *zsize);
+/* Relax the profile: f has rsize * zsize entries; mu receives the chemical potential. */
+int gpe_vortex_relax(const gpe_vortex_params *vp, double *f, double *mu);
+/* Sample psi0 = f(r, z) ((x + i y) / r)^kappa (VortexState::getPsi, VortexState.cpp:204-238) at
+ * every unknown of cfg's grid into a host ABI array (component 0 of ncomp; the others 0). */
+int gpe_vortex_fill(const gpe_config *cfg, const gpe_vortex_params *vp, const double *f, double *psi);
+
 /* ---------------------------------------------------------------- discrete adjoint ---- */
 /* No counterpart in the reference: gradients of a final-state objective with respect to
  * the field-ramp segment values and the trap frequencies, by the discrete adjoint of the

@@ -11,21 +11,75 @@
 * Spinor projections keeping n = |psi|^2: polar / ferromagnetic / mixed
   (Spin1GpeBCC.cu:463-551) and UN / BN / cyclic (Spin2GpeBCC.cu:417-558).
 """
+import ctypes
+
 import numpy as np
 
 from . import lattice as _lattice
 
 
+class _VortexParams(ctypes.Structure):
+    _fields_ = [("kappa", ctypes.c_int), ("g", ctypes.c_double), ("rmin", ctypes.c_double),
+                ("rmax", ctypes.c_double), ("zmax", ctypes.c_double), ("rstep", ctypes.c_double),
+                ("zstep", ctypes.c_double), ("ratio", ctypes.c_double), ("maxiters", ctypes.c_int),
+                ("eps", ctypes.c_double)]
+
+
+def _vortex_lib():
+    from . import gpe
+    lib = gpe.load_library()
+    if not getattr(lib, "_vortex_ready", False):
+        vp = ctypes.POINTER(_VortexParams)
+        dp = ctypes.POINTER(ctypes.c_double)
+        lib.gpe_vortex_size.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        lib.gpe_vortex_relax.argtypes = [vp, dp, dp]
+        lib.gpe_vortex_fill.argtypes = [ctypes.c_void_p, vp, dp, dp]
+        lib._vortex_ready = True
+    return lib
+
+
+def vortex_state(kappa, g, rmax, zmax, rstep, zstep, ratio=1.0, maxiters=20000, eps=1e-29, rmin=0.0):
+    """Axisymmetric stationary vortex f(r, z) and mu by the reference's relaxation
+    (VortexState.cpp:42-196), computed by libgpe's host code (gpe_vortex_relax)."""
+    lib = _vortex_lib()
+    vp = _VortexParams(kappa, g, rmin, rmax, zmax, rstep, zstep, ratio, maxiters, eps)
+    rs, zs = ctypes.c_int(), ctypes.c_int()
+    n = lib.gpe_vortex_size(ctypes.byref(vp), ctypes.byref(rs), ctypes.byref(zs))
+    if n <= 0:
+        raise ValueError("bad vortex profile parameters")
+    f = np.zeros(n)
+    mu = ctypes.c_double()
+    rc = lib.gpe_vortex_relax(ctypes.byref(vp), f.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                              ctypes.byref(mu))
+    if rc != 0:
+        raise ValueError("gpe_vortex_relax failed (%d)" % rc)
+    return dict(params=vp, f=f, mu=mu.value, rsize=rs.value, zsize=zs.value)
+
+
+def vortex_fill(cfg, vs):
+    """psi0 = f(r, z) ((x + i y) / r)^kappa on every unknown of cfg (component 0)."""
+    lib = _vortex_lib()
+    out = np.zeros(cfg.shape, dtype=np.complex128)
+    c = cfg.to_c()
+    rc = lib.gpe_vortex_fill(ctypes.byref(c), ctypes.byref(vs["params"]),
+                             vs["f"].ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                             out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    if rc != 0:
+        raise ValueError("gpe_vortex_fill failed (%d)" % rc)
+    return out
+
+
 def positions(cfg):
-    """Physical positions, shape (nz, ny, nx, V, 3)."""
+    """Physical positions, shape (nz, ny, nx, V, 3): p0 + s (i * block_width + pos_v), the
+    association order of GPBlocks.cpp:487."""
     lat = _lattice.get(cfg.lattice)
     nx, ny, nz = cfg.dims
     s = cfg.block_scale
-    cell = s * lat.block_width
+    w = lat.block_width
     out = np.empty((nz, ny, nx, lat.V, 3))
-    out[..., 0] = cfg.p0[0] + cell * np.arange(nx)[None, None, :, None] + s * lat.pos[None, None, None, :, 0]
-    out[..., 1] = cfg.p0[1] + cell * np.arange(ny)[None, :, None, None] + s * lat.pos[None, None, None, :, 1]
-    out[..., 2] = cfg.p0[2] + cell * np.arange(nz)[:, None, None, None] + s * lat.pos[None, None, None, :, 2]
+    out[..., 0] = cfg.p0[0] + s * (np.arange(nx)[None, None, :, None] * w + lat.pos[None, None, None, :, 0])
+    out[..., 1] = cfg.p0[1] + s * (np.arange(ny)[None, :, None, None] * w + lat.pos[None, None, None, :, 1])
+    out[..., 2] = cfg.p0[2] + s * (np.arange(nz)[:, None, None, None] * w + lat.pos[None, None, None, :, 2])
     return out
 
 
