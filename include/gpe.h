@@ -159,6 +159,29 @@ int gpe_time(const gpe_ctx *ctx, double *t, long long *sweeps);
  * sweep (3 touches x ncomp x 2 x sizeof(real) per unknown). */
 int gpe_last_step_timing(const gpe_ctx *ctx, double *ms_per_sweep, double *bytes_per_sweep);
 
+/* ---------------------------------------------------------------------- ensembles ---- */
+/* No counterpart in the reference (one executable per run there): many independent runs,
+ * each an ordinary started context (own lattice, size, dt, couplings, ramp), advanced
+ * together with ONE kernel launch per sweep for all members.  Members must share ncomp,
+ * precision and scheme and live on the current device. */
+typedef struct gpe_ensemble gpe_ensemble;
+
+int gpe_ensemble_create(gpe_ctx *const *members, int n, gpe_ensemble **out);
+int gpe_ensemble_destroy(gpe_ensemble *ens);
+const char *gpe_ensemble_last_error(const gpe_ensemble *ens);
+/* Advance every member by nsteps.  graph_sweeps > 0 (even, dividing the sweep count)
+ * replays the launches from a captured hipGraph of graph_sweeps sweeps; 0 launches eagerly. */
+int gpe_ensemble_step(gpe_ensemble *ens, int nsteps, int graph_sweeps);
+/* Average ms per batched sweep of the last gpe_ensemble_step, and total unknowns. */
+int gpe_ensemble_timing(const gpe_ensemble *ens, double *ms_per_sweep, double *unknowns);
+
+/* -------------------------------------------------------------------- autotuning ---- */
+/* Time the traversal candidates (y-tile rows 1..32 x XCD remap on/off) on scratch copies
+ * of the context's two levels for trial_sweeps sweeps each, and keep the fastest.  Results
+ * are cached per (lattice, ncomp, precision, nx, ny, nz) for the process; pass
+ * trial_sweeps = 0 to apply a cached choice only.  Outputs may be NULL. */
+int gpe_autotune(gpe_ctx *ctx, int trial_sweeps, int *tile_rows, int *xcd_remap, double *ms_per_sweep);
+
 /* ---------------------------------------------------------------- initial states ---- */
 /* Stationary vortex state f(r, z) e^{i kappa phi} of the scalar GPE in the trap
  * 0.5 (r^2 + ratio^2 z^2), by the reference's axisymmetric relaxation

@@ -54,11 +54,12 @@ struct WaveCoord {
     bool valid;
 };
 
-__device__ __forceinline__ WaveCoord decode(const Geometry &g)
+// Wave coordinates of workgroup b of nb (b, nb: the workgroup's place in its own run).
+__device__ __forceinline__ WaveCoord decode(const Geometry &g, int b, int nb)
 {
     WaveCoord wc;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const long long w = (long long)logical_block(blockIdx.x, gridDim.x) * 4 + wid;
+    const long long w = (long long)(g.xcd_remap ? logical_block(b, nb) : b) * 4 + wid;
     long long r = w;
     const int xseg = (int)(r % g.nseg);
     r /= g.nseg;
@@ -73,6 +74,8 @@ __device__ __forceinline__ WaveCoord decode(const Geometry &g)
     wc.x = xseg * 64 + (int)(threadIdx.x & 63);
     return wc;
 }
+
+__device__ __forceinline__ WaveCoord decode(const Geometry &g) { return decode(g, blockIdx.x, gridDim.x); }
 
 __device__ __forceinline__ long long line_index(const Geometry &g, int c, int z, int y, int v)
 {

@@ -9,8 +9,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
 #include <string>
+#include <tuple>
 #include <vector>
 
 using namespace gpe;
@@ -271,6 +274,7 @@ int gpe_create(const gpe_config *cfg, gpe_ctx **out)
         return GPE_EINVAL;
     }
     g.blocks = (int)blocks;
+    g.xcd_remap = 1;
     c->level_elems = (long long)cfg->ncomp * g.nz * g.ny * g.V * g.lx;
 
     Physics &ph = c->ph;
@@ -916,6 +920,255 @@ int gpe_adjoint(gpe_ctx *c, const gpe_adjoint_spec *spec, gpe_adjoint_result *re
         }
     }
 #undef ADJ_CHECK
+    return GPE_OK;
+}
+
+// ------------------------------------------------------------------------- ensembles ---
+
+struct gpe_ensemble {
+    std::vector<gpe_ctx *> m;
+    std::vector<EnsembleMember> h;
+    EnsembleMember *d = nullptr;
+    int *d_base = nullptr;
+    int blocks = 0, ncomp = 0, fp64 = 0, per = 1;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    int graph_sweeps = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0, unknowns = 0.0;
+    std::string err;
+};
+
+static int ens_fail(gpe_ensemble *e, int code, const std::string &what)
+{
+    if (e) e->err = what;
+    return code;
+}
+
+int gpe_ensemble_create(gpe_ctx *const *members, int n, gpe_ensemble **out)
+{
+    if (!members || n < 1 || !out) return GPE_EINVAL;
+    *out = nullptr;
+    gpe_ctx *c0 = members[0];
+    for (int i = 0; i < n; ++i) {
+        gpe_ctx *c = members[i];
+        if (!c) return GPE_EINVAL;
+        if (c->cfg.ncomp != c0->cfg.ncomp || c->fp64 != c0->fp64 || c->cfg.scheme != c0->cfg.scheme ||
+            c->device != c0->device)
+            return GPE_EINVAL;
+        if (!c->level1_valid) return GPE_ESTATE;
+    }
+    gpe_ensemble *e = new (std::nothrow) gpe_ensemble();
+    if (!e) return GPE_ENOMEM;
+    e->m.assign(members, members + n);
+    e->ncomp = c0->cfg.ncomp;
+    e->fp64 = c0->fp64;
+    e->per = c0->cfg.scheme == GPE_SCHEME_STAGGERED ? 2 : 1;
+    e->h.resize(n);
+    int begin = 0;
+    for (int i = 0; i < n; ++i) {
+        e->h[i].begin = begin;
+        e->h[i].blocks = members[i]->g.blocks;
+        begin += (members[i]->g.blocks + 7) / 8 * 8;
+        e->unknowns += (double)members[i]->cfg.nx * members[i]->cfg.ny * members[i]->cfg.nz * members[i]->g.V;
+    }
+    e->blocks = begin;
+    int rc = GPE_OK;
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) rc = GPE_EDEVICE;
+    if (!rc && hipMalloc((void **)&e->d, sizeof(EnsembleMember) * n) != hipSuccess) rc = GPE_ENOMEM;
+    if (!rc && hipMalloc((void **)&e->d_base, sizeof(int) * 4) != hipSuccess) rc = GPE_ENOMEM;
+    if (!rc && (hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess)) rc = GPE_EDEVICE;
+    if (rc) {
+        gpe_ensemble_destroy(e);
+        return rc;
+    }
+    *out = e;
+    return GPE_OK;
+}
+
+int gpe_ensemble_destroy(gpe_ensemble *e)
+{
+    if (!e) return GPE_EINVAL;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    if (e->d) (void)hipFree(e->d);
+    if (e->d_base) (void)hipFree(e->d_base);
+    if (e->ev0) (void)hipEventDestroy(e->ev0);
+    if (e->ev1) (void)hipEventDestroy(e->ev1);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return GPE_OK;
+}
+
+const char *gpe_ensemble_last_error(const gpe_ensemble *e) { return e ? e->err.c_str() : "null ensemble"; }
+
+#define ENSCALL(e, call)                                                                      \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess) return ens_fail(e, GPE_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int gpe_ensemble_step(gpe_ensemble *e, int nsteps, int graph_sweeps)
+{
+    if (!e) return GPE_EINVAL;
+    if (nsteps < 0) return ens_fail(e, GPE_EINVAL, "negative step count");
+    const int nsw = nsteps * e->per;
+    if (nsw == 0) return GPE_OK;
+    if (graph_sweeps < 0 || (graph_sweeps > 0 && (graph_sweeps % 2 != 0 || nsw % graph_sweeps != 0)))
+        return ens_fail(e, GPE_EINVAL, "graph_sweeps must be even and divide the sweep count");
+    for (size_t i = 0; i < e->m.size(); ++i) {
+        gpe_ctx *c = e->m[i];
+        if (!c->level1_valid) return ens_fail(e, GPE_ESTATE, "member not started");
+        int rc = upload_fields(c, c->sweeps, nsw);
+        if (rc) return ens_fail(e, rc, c->err);
+        ENSCALL(e, hipStreamSynchronize(c->stream));
+        const double coef = 2.0 * sweep_dt(c);
+        e->h[i].args[0] = sweep_args(c, 0, c->level[0], c->level[1], coef);
+        e->h[i].args[1] = sweep_args(c, 0, c->level[1], c->level[0], coef);
+    }
+    ENSCALL(e, hipMemcpyAsync(e->d, e->h.data(), sizeof(EnsembleMember) * e->h.size(), hipMemcpyHostToDevice,
+                              e->stream));
+    EnsembleArgs a;
+    a.members = e->d;
+    a.nmember = (int)e->h.size();
+    ENSCALL(e, hipEventRecord(e->ev0, e->stream));
+    if (graph_sweeps > 0) {
+        if (!e->gexec || e->graph_sweeps != graph_sweeps) {
+            if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+            if (e->graph) (void)hipGraphDestroy(e->graph);
+            e->gexec = nullptr;
+            e->graph = nullptr;
+            ENSCALL(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < graph_sweeps; ++k) {
+                a.parity = k & 1;
+                a.sweep = k;
+                a.sweep_base = e->d_base;
+                (void)launch_ensemble_sweep(e->fp64, e->ncomp, a, e->blocks, e->stream);
+            }
+            (void)launch_advance_base(e->d_base, graph_sweeps, e->stream);
+            ENSCALL(e, hipStreamEndCapture(e->stream, &e->graph));
+            ENSCALL(e, hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
+            e->graph_sweeps = graph_sweeps;
+        }
+        ENSCALL(e, launch_set_base(e->d_base, 0, e->stream));
+        for (int r = 0; r < nsw / graph_sweeps; ++r) ENSCALL(e, hipGraphLaunch(e->gexec, e->stream));
+    } else {
+        for (int k = 0; k < nsw; ++k) {
+            a.parity = k & 1;
+            a.sweep = k;
+            a.sweep_base = nullptr;
+            ENSCALL(e, launch_ensemble_sweep(e->fp64, e->ncomp, a, e->blocks, e->stream));
+        }
+    }
+    ENSCALL(e, hipEventRecord(e->ev1, e->stream));
+    ENSCALL(e, hipEventSynchronize(e->ev1));
+    float ms = 0.f;
+    ENSCALL(e, hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    e->last_ms = ms / nsw;
+    for (gpe_ctx *c : e->m) {
+        if (nsw & 1) std::swap(c->level[0], c->level[1]);
+        c->sweeps += nsw;
+    }
+    return GPE_OK;
+}
+
+int gpe_ensemble_timing(const gpe_ensemble *e, double *ms_per_sweep, double *unknowns)
+{
+    if (!e) return GPE_EINVAL;
+    if (ms_per_sweep) *ms_per_sweep = e->last_ms;
+    if (unknowns) *unknowns = e->unknowns;
+    return GPE_OK;
+}
+
+// ------------------------------------------------------------------------ autotuning ---
+
+static void set_traversal(gpe_ctx *c, int ty, int remap)
+{
+    Geometry &g = c->g;
+    g.ty = ty < g.ny ? ty : g.ny;
+    g.ntile = (g.ny + g.ty - 1) / g.ty;
+    g.waves = (long long)g.ntile * g.ty * g.nz * g.V * g.nseg;
+    g.blocks = (int)((g.waves + 3) / 4);
+    g.xcd_remap = remap;
+}
+
+typedef std::tuple<int, int, int, int, int, int> TuneKey;
+static std::map<TuneKey, std::pair<int, int>> g_tune_cache;
+static std::mutex g_tune_mutex;
+
+int gpe_autotune(gpe_ctx *c, int trial_sweeps, int *tile_rows, int *xcd_remap, double *ms_out)
+{
+    if (!c) return GPE_EINVAL;
+    if (trial_sweeps < 0) return fail(c, GPE_EINVAL, "negative trial sweeps");
+    const TuneKey key(c->cfg.lattice, c->cfg.ncomp, c->fp64, c->cfg.nx, c->cfg.ny, c->cfg.nz);
+    const int max_blocks_before = c->g.blocks;
+    int best_ty = c->g.ty, best_remap = c->g.xcd_remap;
+    double best_ms = -1.0;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mutex);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            best_ty = it->second.first;
+            best_remap = it->second.second;
+            trial_sweeps = 0;
+        }
+    }
+    if (trial_sweeps > 0) {
+        const size_t lbytes = (size_t)c->level_elems * c->elem_bytes;
+        void *s0 = nullptr, *s1 = nullptr;
+        double *parts = nullptr;
+        if (hipMalloc(&s0, lbytes) != hipSuccess || hipMalloc(&s1, lbytes) != hipSuccess) {
+            if (s0) (void)hipFree(s0);
+            return fail(c, GPE_ENOMEM, "autotune scratch does not fit");
+        }
+        HIPCALL(c, hipMemcpyAsync(s0, c->level[0], lbytes, hipMemcpyDeviceToDevice, c->stream));
+        HIPCALL(c, hipMemcpyAsync(s1, c->level[1], lbytes, hipMemcpyDeviceToDevice, c->stream));
+        int rc = upload_fields(c, c->sweeps, 1);
+        if (rc) return rc;
+        const int tys[] = {1, 2, 4, 8, 16, 32};
+        for (int ty : tys) {
+            if (ty > c->g.ny && ty != 1) continue;
+            for (int remap = 0; remap < 2; ++remap) {
+                set_traversal(c, ty, remap);
+                void *cur = s0, *nxt = s1;
+                for (int k = 0; k < trial_sweeps + 2; ++k) {
+                    if (k == 2) HIPCALL(c, hipEventRecord(c->ev0, c->stream));
+                    SweepArgs a = sweep_args(c, 0, cur, nxt, 2.0 * sweep_dt(c));
+                    hipError_t e = launch_sweep(c->fp64, c->cfg.ncomp, MODE_UPDATE, a, c->stream);
+                    if (e != hipSuccess) return hip_fail(c, e, "autotune sweep");
+                    std::swap(cur, nxt);
+                }
+                HIPCALL(c, hipEventRecord(c->ev1, c->stream));
+                HIPCALL(c, hipEventSynchronize(c->ev1));
+                float ms = 0.f;
+                HIPCALL(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+                const double per = ms / trial_sweeps;
+                if (best_ms < 0 || per < best_ms) {
+                    best_ms = per;
+                    best_ty = c->g.ty;
+                    best_remap = remap;
+                }
+            }
+        }
+        (void)hipFree(s0);
+        (void)hipFree(s1);
+        (void)parts;
+        std::lock_guard<std::mutex> lk(g_tune_mutex);
+        g_tune_cache[key] = std::make_pair(best_ty, best_remap);
+    }
+    set_traversal(c, best_ty, best_remap);
+    if (c->g.blocks > max_blocks_before) {  // partials are sized by workgroups
+        HIPCALL(c, hipStreamSynchronize(c->stream));
+        (void)hipFree(c->d_partials);
+        c->d_partials = nullptr;
+        if (hipMalloc((void **)&c->d_partials, sizeof(double) * OBS_COUNT * (size_t)c->g.blocks) != hipSuccess)
+            return fail(c, GPE_ENOMEM, "partials");
+    }
+    if (tile_rows) *tile_rows = best_ty;
+    if (xcd_remap) *xcd_remap = best_remap;
+    if (ms_out) *ms_out = best_ms;
     return GPE_OK;
 }
 

@@ -23,6 +23,7 @@ struct Geometry {
     long long lx;      // elements per x-line
     long long waves;   // total waves of a sweep
     int blocks;        // workgroups of a sweep (4 waves each)
+    int xcd_remap;     // give each XCD a contiguous range of the traversal order
 };
 
 // Per-context constant tables on the device (weights already scaled by -1/2 / s^2).
@@ -119,6 +120,25 @@ struct ObjArgs {
 
 hipError_t launch_adjoint(int fp64, int ncomp, const AdjArgs &a, hipStream_t s);
 hipError_t launch_objective(int fp64, int ncomp, const ObjArgs &a, hipStream_t s);
+
+// Ensemble: one launch advances every member by one sweep.
+struct EnsembleMember {
+    SweepArgs args[2];        // parity 0: cur = level 0, next = level 1; parity 1: swapped
+    int begin;                // first workgroup (multiple of 8)
+    int blocks;               // workgroups of this member
+};
+
+struct EnsembleArgs {
+    const EnsembleMember *members;
+    int nmember;
+    int parity;
+    int sweep;                // sweep index within the uploaded field tables
+    const int *sweep_base;    // device counter added to sweep (graph replays), may be null
+};
+
+hipError_t launch_ensemble_sweep(int fp64, int ncomp, const EnsembleArgs &e, int blocks, hipStream_t s);
+hipError_t launch_advance_base(int *base, int by, hipStream_t s);
+hipError_t launch_set_base(int *base, int value, hipStream_t s);
 
 // launchers (gpe_kernels.hip)
 hipError_t launch_sweep(int fp64, int ncomp, int mode, const SweepArgs &a, hipStream_t s);

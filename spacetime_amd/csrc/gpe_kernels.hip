@@ -20,11 +20,11 @@ namespace gpe {
 // ------------------------------------------------------------------------- the sweep --
 
 template <typename R, int C, int DEG, int MODE>
-__global__ __launch_bounds__(256) void k_sweep(SweepArgs a)
+__device__ __forceinline__ void sweep_body(const SweepArgs &a, int sweep, int b, int nb)
 {
     using V2 = vec2_t<R>;
     const Geometry &g = a.g;
-    const WaveCoord wc = decode(g);
+    const WaveCoord wc = decode(g, b, nb);
     double part = 0.0;
 
     if (wc.valid) {
@@ -103,7 +103,7 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a)
         }
         // position, trap and field
         const Physics &ph = a.ph;
-        const FieldRec &fr = a.fields[a.sweep];
+        const FieldRec &fr = a.fields[sweep];
         const R *pos = table_pos<R>(a.tb) + 3 * v;
         const R cell = (R)ph.cell;
         const R rx = (R)ph.p0[0] + cell * (R)x + pos[0];
@@ -157,8 +157,51 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a)
         part = wave_sum(part);
         if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
         __syncthreads();
-        if (threadIdx.x == 0) a.partials[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+        if (threadIdx.x == 0) a.partials[b] = (red[0] + red[1]) + (red[2] + red[3]);
     }
+}
+
+template <typename R, int C, int DEG, int MODE>
+__global__ __launch_bounds__(256) void k_sweep(SweepArgs a)
+{
+    sweep_body<R, C, DEG, MODE>(a, a.sweep, blockIdx.x, gridDim.x);
+}
+
+// One leapfrog sweep of every member of an ensemble in a single launch.  Member m owns
+// workgroups [begin_m, begin_m + blocks_m) (begin_m a multiple of 8, so the XCD remap of the
+// member's own range stays valid); its arguments for parity p are m.args[p].
+template <typename R, int C>
+__global__ __launch_bounds__(256) void k_ensemble_sweep(EnsembleArgs e)
+{
+    const int b = blockIdx.x;
+    int lo = 0, hi = e.nmember - 1;
+    while (lo < hi) {  // last member with begin <= b
+        const int mid = (lo + hi + 1) >> 1;
+        if (e.members[mid].begin <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const EnsembleMember &m = e.members[lo];
+    const int local = b - m.begin;
+    if (local >= m.blocks) return;  // padding workgroup (uniform)
+    const SweepArgs &a = m.args[e.parity];
+    const int sweep = e.sweep + (e.sweep_base ? *e.sweep_base : 0);
+    switch (a.g.deg) {
+    case 4: sweep_body<R, C, 4, MODE_UPDATE>(a, sweep, local, m.blocks); break;
+    case 6: sweep_body<R, C, 6, MODE_UPDATE>(a, sweep, local, m.blocks); break;
+    default: sweep_body<R, C, 8, MODE_UPDATE>(a, sweep, local, m.blocks); break;
+    }
+}
+
+__global__ void k_advance_base(int *base, int by)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *base += by;
+}
+
+__global__ void k_set_base(int *base, int value)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *base = value;
 }
 
 // ------------------------------------------------------------------------ observables --
@@ -464,6 +507,36 @@ static hipError_t obs_comp(int ncomp, const ObsArgs &a, hipStream_t s)
 hipError_t launch_observables(int fp64, int ncomp, const ObsArgs &a, hipStream_t s)
 {
     return fp64 ? obs_comp<double>(ncomp, a, s) : obs_comp<float>(ncomp, a, s);
+}
+
+template <typename R>
+static hipError_t ens_comp(int ncomp, const EnsembleArgs &e, int blocks, hipStream_t s)
+{
+    dim3 grid(blocks), block(256);
+    switch (ncomp) {
+    case 1: hipLaunchKernelGGL((k_ensemble_sweep<R, 1>), grid, block, 0, s, e); break;
+    case 3: hipLaunchKernelGGL((k_ensemble_sweep<R, 3>), grid, block, 0, s, e); break;
+    case 5: hipLaunchKernelGGL((k_ensemble_sweep<R, 5>), grid, block, 0, s, e); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_ensemble_sweep(int fp64, int ncomp, const EnsembleArgs &e, int blocks, hipStream_t s)
+{
+    return fp64 ? ens_comp<double>(ncomp, e, blocks, s) : ens_comp<float>(ncomp, e, blocks, s);
+}
+
+hipError_t launch_advance_base(int *base, int by, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_advance_base, dim3(1), dim3(64), 0, s, base, by);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_base(int *base, int value, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_set_base, dim3(1), dim3(64), 0, s, base, value);
+    return hipGetLastError();
 }
 
 hipError_t launch_finalize(const double *partials, int n, int width, int maxcol, double *out, hipStream_t s)

@@ -109,6 +109,13 @@ def load_library(path=LIB_PATH):
     lib.gpe_time.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_longlong)]
     lib.gpe_last_step_timing.argtypes = [vp, _dp, _dp]
     lib.gpe_adjoint.argtypes = [vp, ctypes.POINTER(_AdjSpec), ctypes.POINTER(_AdjRes)]
+    lib.gpe_ensemble_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)]
+    lib.gpe_ensemble_destroy.argtypes = [vp]
+    lib.gpe_ensemble_last_error.argtypes = [vp]
+    lib.gpe_ensemble_last_error.restype = ctypes.c_char_p
+    lib.gpe_ensemble_step.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    lib.gpe_ensemble_timing.argtypes = [vp, _dp, _dp]
+    lib.gpe_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _dp]
     lib.gpe_set_trap.argtypes = [vp, _dp]
     _LIB = lib
     return lib
@@ -321,8 +328,60 @@ class Stepper:
         self._check(self.lib.gpe_time(self._h, ctypes.byref(t), ctypes.byref(n)), "gpe_time")
         return t.value, n.value
 
+    def autotune(self, trial_sweeps=20):
+        """Pick the fastest traversal (y-tile rows, XCD remap) for this context (gpe_autotune)."""
+        ty, rm = ctypes.c_int(), ctypes.c_int()
+        ms = ctypes.c_double()
+        self._check(self.lib.gpe_autotune(self._h, int(trial_sweeps), ctypes.byref(ty), ctypes.byref(rm),
+                                          ctypes.byref(ms)), "gpe_autotune")
+        return dict(tile_rows=ty.value, xcd_remap=rm.value, ms_per_sweep=ms.value)
+
     def last_step_timing(self):
         ms = ctypes.c_double()
         by = ctypes.c_double()
         self._check(self.lib.gpe_last_step_timing(self._h, ctypes.byref(ms), ctypes.byref(by)), "timing")
         return ms.value, by.value
+
+
+class Ensemble:
+    """Independent started Steppers advanced together, one launch per sweep for all of them
+    (gpe_ensemble_*).  The member Steppers stay usable for set/get state and observables."""
+
+    def __init__(self, members):
+        self.lib = load_library()
+        self.members = list(members)
+        arr = (ctypes.c_void_p * len(self.members))(*[m._h.value for m in self.members])
+        h = ctypes.c_void_p()
+        rc = self.lib.gpe_ensemble_create(arr, len(self.members), ctypes.byref(h))
+        if rc != 0:
+            raise GpeError("gpe_ensemble_create failed: %s (%d)" % (ERRORS.get(rc, "?"), rc))
+        self._h = h
+
+    def step(self, nsteps, graph_sweeps=0):
+        rc = self.lib.gpe_ensemble_step(self._h, int(nsteps), int(graph_sweeps))
+        if rc != 0:
+            msg = self.lib.gpe_ensemble_last_error(self._h)
+            raise GpeError("gpe_ensemble_step failed: %s (%d): %s" % (ERRORS.get(rc, "?"), rc,
+                                                                     msg.decode() if msg else ""))
+
+    def timing(self):
+        ms, n = ctypes.c_double(), ctypes.c_double()
+        self.lib.gpe_ensemble_timing(self._h, ctypes.byref(ms), ctypes.byref(n))
+        return ms.value, n.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.gpe_ensemble_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
