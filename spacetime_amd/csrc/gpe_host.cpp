@@ -644,6 +644,26 @@ int gpe_time(const gpe_ctx *c, double *t, long long *sweeps)
     return GPE_OK;
 }
 
+int gpe_fill_thomas_fermi(gpe_ctx *c, double g, double mu, const double *weights, double noise, unsigned long long seed,
+                          double t)
+{
+    if (!c) return GPE_EINVAL;
+    if (!(g > 0.0) || !weights || noise < 0.0) return fail(c, GPE_EINVAL, "bad Thomas-Fermi fill arguments");
+    FillArgs f;
+    f.mu = mu;
+    f.g = g;
+    f.amp = noise;
+    f.seed = seed;
+    for (int k = 0; k < 10; ++k) f.w[k] = k < 2 * c->cfg.ncomp ? weights[k] : 0.0;
+    hipError_t e = launch_fill_tf(c->fp64, c->g, tables(c), c->ph, c->cfg.ncomp, f, c->level[0], c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "fill");
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+    c->t0 = t;
+    c->sweeps = 0;
+    c->level1_valid = false;
+    return GPE_OK;
+}
+
 int gpe_set_trap(gpe_ctx *c, const double trap[3])
 {
     if (!c || !trap) return GPE_EINVAL;

@@ -140,6 +140,15 @@ hipError_t launch_ensemble_sweep(int fp64, int ncomp, const EnsembleArgs &e, int
 hipError_t launch_advance_base(int *base, int by, hipStream_t s);
 hipError_t launch_set_base(int *base, int value, hipStream_t s);
 
+// Device-side Thomas-Fermi fill with spinor weights and counter-based noise.
+struct FillArgs {
+    double mu, g, amp;
+    unsigned long long seed;
+    double w[10];             // complex weight of each component
+};
+hipError_t launch_fill_tf(int fp64, const Geometry &g, const Tables &tb, const Physics &ph, int ncomp, const FillArgs &f,
+                          void *psi, hipStream_t s);
+
 // launchers (gpe_kernels.hip)
 hipError_t launch_sweep(int fp64, int ncomp, int mode, const SweepArgs &a, hipStream_t s);
 hipError_t launch_observables(int fp64, int ncomp, const ObsArgs &a, hipStream_t s);

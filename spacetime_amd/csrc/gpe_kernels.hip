@@ -405,6 +405,68 @@ __global__ __launch_bounds__(256) void k_phase_copy(const vec2_t<R> *src, vec2_t
     dst[i] = o;
 }
 
+// Counter-based uniform variate in [0, 1) from (seed, index, stream): a 64-bit mix
+// (splitmix64 finaliser) of the counter, so every unknown draws independently of the
+// traversal and of the launch shape.
+__device__ __forceinline__ double hash_uniform(unsigned long long seed, unsigned long long idx, unsigned s)
+{
+    unsigned long long z = seed * 0x9E3779B97F4A7C15ull + idx * 0xD1B54A32D192ED03ull + (unsigned long long)s * 0x94D049BB133111EBull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// psi_c = w_c sqrt(max(mu - V, 0) / g) (1 + amp * eps_c), eps_c uniform in the unit disk.
+template <typename R>
+__global__ __launch_bounds__(256) void k_fill_tf(Geometry g, Tables tb, Physics ph, int ncomp, FillArgs f, void *psi)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long total = (long long)g.nz * g.ny * g.V * g.nx;
+    if (i >= total) return;
+    long long r = i;
+    const int x = (int)(r % g.nx);
+    r /= g.nx;
+    const int v = (int)(r % g.V);
+    r /= g.V;
+    const int y = (int)(r % g.ny);
+    const int z = (int)(r / g.ny);
+    const double *pos = tb.pos64 + 3 * v;
+    const double rx = ph.p0[0] + ph.cell * x + pos[0];
+    const double ry = ph.p0[1] + ph.cell * y + pos[1];
+    const double rz = ph.p0[2] + ph.cell * z + pos[2];
+    const double V = 0.5 * (ph.trap2[0] * rx * rx + ph.trap2[1] * ry * ry + ph.trap2[2] * rz * rz) + ph.trap_offset;
+    const double a = sqrt(fmax(f.mu - V, 0.0) / f.g);
+    const unsigned long long node = (((unsigned long long)z * g.ny + y) * g.nx + x) * g.V + v;
+    vec2_t<R> *out = (vec2_t<R> *)psi;
+    for (int c = 0; c < ncomp; ++c) {
+        double er = 0, ei = 0;
+        if (f.amp != 0.0) {
+            const double rad = sqrt(hash_uniform(f.seed, node, 2 * c));
+            const double ang = 6.283185307179586 * hash_uniform(f.seed, node, 2 * c + 1);
+            er = rad * cos(ang);
+            ei = rad * sin(ang);
+        }
+        const double br = a * (1.0 + f.amp * er), bi = a * f.amp * ei;  // a (1 + amp eps)
+        vec2_t<R> o;
+        o.x = (R)(f.w[2 * c] * br - f.w[2 * c + 1] * bi);
+        o.y = (R)(f.w[2 * c] * bi + f.w[2 * c + 1] * br);
+        out[line_index(g, c, z, y, v) + x] = o;
+    }
+}
+
+hipError_t launch_fill_tf(int fp64, const Geometry &g, const Tables &tb, const Physics &ph, int ncomp, const FillArgs &f,
+                          void *psi, hipStream_t s)
+{
+    const long long total = (long long)g.nz * g.ny * g.V * g.nx;
+    const unsigned grid = (unsigned)((total + 255) / 256);
+    if (fp64)
+        hipLaunchKernelGGL(k_fill_tf<double>, dim3(grid), dim3(256), 0, s, g, tb, ph, ncomp, f, psi);
+    else
+        hipLaunchKernelGGL(k_fill_tf<float>, dim3(grid), dim3(256), 0, s, g, tb, ph, ncomp, f, psi);
+    return hipGetLastError();
+}
+
 // ABI index (c, z, y, x, v) <-> device index (c, z, y, v, x)
 template <typename R, bool PACK>
 __global__ __launch_bounds__(256) void k_pack(Geometry g, int ncomp, const double2 *abi_in, double2 *abi_out,

@@ -117,6 +117,8 @@ def load_library(path=LIB_PATH):
     lib.gpe_ensemble_timing.argtypes = [vp, _dp, _dp]
     lib.gpe_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _dp]
     lib.gpe_set_trap.argtypes = [vp, _dp]
+    lib.gpe_fill_thomas_fermi.argtypes = [vp, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_double,
+                                          ctypes.c_ulonglong, ctypes.c_double]
     _LIB = lib
     return lib
 
@@ -240,6 +242,19 @@ class Stepper:
 
     def get_state_device(self, ptr, level=0):
         self._check(self.lib.gpe_get_state_device(self._h, level, ctypes.c_void_p(ptr)), "gpe_get_state_device")
+
+    def fill_thomas_fermi(self, g, mu, weights=None, noise=0.0, seed=0, t=0.0):
+        """Level 0 := w_c sqrt(max(mu - V, 0) / g) (1 + noise eps_c) on the device
+        (gpe_fill_thomas_fermi); weights default to all weight in component 0."""
+        C = self.config.ncomp
+        w = np.zeros(C, dtype=np.complex128)
+        if weights is None:
+            w[0] = 1.0
+        else:
+            w[:] = weights
+        wd = np.ascontiguousarray(np.stack([w.real, w.imag], axis=1).reshape(-1))
+        self._check(self.lib.gpe_fill_thomas_fermi(self._h, g, mu, wd.ctypes.data_as(_dp), noise, seed, t),
+                    "gpe_fill_thomas_fermi")
 
     # -- fields --------------------------------------------------------------------------------
     def set_field(self, bq=0.0, bb=(0.0, 0.0, 0.0), bq_quad=0.0, bb_quad=(0.0, 0.0, 0.0)):

@@ -200,3 +200,28 @@ def knot_ramp(knots, duration):
     segs = [((0.0, 0.0, float(knots[0])), 0.0, 0)]
     segs += [((0.0, 0.0, float(v)), duration / (k - 1), 1) for v in knots[1:]]
     return segs
+
+
+def read_reference_dat(path, dims, ncomp, V=12):
+    """Load a state file of the reference solvers into the ABI layout (ncomp, nz, ny, nx, V).
+
+    The files are raw dumps of the padded array of blocks written with ofstream::write
+    (Spin1GpeBCC.cu:1464-1467,1690-1699; read back at :1228-1237): (nz+2)(ny+2)(nx+2) blocks,
+    block (i, j, k) at index (k+1)(ny+2)(nx+2) + (j+1)(nx+2) + (i+1), each block holding V
+    unknowns of ncomp complex doubles ordered m = F..-F (Complex3Vec s1, s0, s_1;
+    Complex5Vec s2..s_2; the scalar solvers' double2)."""
+    nx, ny, nz = dims
+    count = (nx + 2) * (ny + 2) * (nz + 2) * V * ncomp
+    raw = np.fromfile(path, dtype=np.complex128)
+    if raw.size < count:
+        raise ValueError("%s holds %d complex values, expected %d" % (path, raw.size, count))
+    blocks = raw[:count].reshape(nz + 2, ny + 2, nx + 2, V, ncomp)
+    return np.ascontiguousarray(blocks[1:-1, 1:-1, 1:-1].transpose(4, 0, 1, 2, 3))
+
+
+def write_reference_dat(path, psi):
+    """Write an ABI-layout state as the reference's padded block file (zero halo)."""
+    ncomp, nz, ny, nx, V = psi.shape
+    blocks = np.zeros((nz + 2, ny + 2, nx + 2, V, ncomp), dtype=np.complex128)
+    blocks[1:-1, 1:-1, 1:-1] = psi.transpose(1, 2, 3, 4, 0)
+    blocks.tofile(path)

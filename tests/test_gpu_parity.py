@@ -207,3 +207,44 @@ def test_errors_are_reported():
         assert not o.finite
     with pytest.raises(gpe.GpeError, match="EINVAL"):
         gpe.Stepper(gpe.Config(lattice="bcc", dims=(0, 4, 4)))
+
+
+def _hash_uniform(seed, idx, s):
+    """Host restatement of the fill's counter-based variate (splitmix64 finaliser)."""
+    M = np.uint64(0xFFFFFFFFFFFFFFFF)
+    with np.errstate(over="ignore"):
+        z = (np.uint64(seed) * np.uint64(0x9E3779B97F4A7C15) + idx.astype(np.uint64) * np.uint64(0xD1B54A32D192ED03)
+             + np.uint64(s) * np.uint64(0x94D049BB133111EB)) & M
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) / 2.0 ** 53
+
+
+@pytest.mark.parametrize("lattice_name,ncomp,precision", [("bcc", 5, 1), ("c15", 3, 1), ("fcc", 5, 0)])
+def test_device_thomas_fermi_fill(lattice_name, ncomp, precision):
+    from spacetime_amd import states
+    cfg = _config(lattice_name, ncomp, precision, (6, 5, 4), cell=0.5)
+    r = states.positions(cfg)
+    V = states.trap_potential(cfg, r)
+    g, mu = 40.0, float(np.median(V))
+    tf = np.sqrt(np.maximum(mu - V, 0.0) / g)
+    w = np.exp(1j * np.arange(ncomp)) / np.sqrt(ncomp)
+    nz, ny, nx, nv = V.shape
+    node = np.arange(nz * ny * nx * nv).reshape(nz, ny, nx, nv)
+    expect = np.empty(cfg.shape, dtype=np.complex128)
+    amp, seed = 0.01, 54363
+    for c in range(ncomp):
+        rad = np.sqrt(_hash_uniform(seed, node, 2 * c))
+        eps = rad * np.exp(2j * np.pi * _hash_uniform(seed, node, 2 * c + 1))
+        expect[c] = w[c] * tf * (1.0 + amp * eps)
+    tol = 1e-14 if precision == 1 else 1e-6
+    with gpe.Stepper(cfg) as st:
+        st.fill_thomas_fermi(g, mu, w, noise=0.0, t=0.25)
+        clean = st.get_state(0)
+        assert st.time()[0] == pytest.approx(0.25)
+        st.fill_thomas_fermi(g, mu, w, noise=amp, seed=seed)
+        noisy = st.get_state(0)
+    assert rel_l2(clean, w[:, None, None, None, None] * tf[None]) < tol
+    assert rel_l2(noisy, expect) < tol
+    assert 0.0 < rel_l2(noisy, clean) < amp

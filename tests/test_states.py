@@ -89,3 +89,22 @@ def test_imprints_noise_and_projections():
     # cyclic state has zero spin density, ferro has |F| = 2 n
     cyc = states.project(a, "cyclic")
     assert np.allclose(np.abs(cyc[0]) ** 2 * 3, n) and np.allclose(np.abs(cyc[3]) ** 2 * 1.5, n)
+
+
+def test_reference_dat_roundtrip_and_layout(tmp_path):
+    rng = np.random.default_rng(9)
+    dims = (3, 4, 2)
+    psi = rng.normal(size=(3, 2, 4, 3, 12)) + 1j * rng.normal(size=(3, 2, 4, 3, 12))
+    path = tmp_path / "state.dat"
+    states.write_reference_dat(str(path), psi)
+    raw = np.fromfile(str(path), dtype=np.float64)
+    assert raw.size == 5 * 6 * 4 * 12 * 3 * 2
+    # block (i, j, k) = (2, 1, 0), unknown 7, component m = 0 sits where the reference puts it:
+    # dstI = (k+1)(ny+2)(nx+2) + (j+1)(nx+2) + (i+1); Complex3Vec {s1, s0, s_1}
+    i, j, k, l = 2, 1, 0, 7
+    dst = (k + 1) * 6 * 5 + (j + 1) * 5 + (i + 1)
+    off = ((dst * 12 + l) * 3 + 1) * 2
+    assert raw[off] == psi[1, k, j, i, l].real and raw[off + 1] == psi[1, k, j, i, l].imag
+    assert np.all(raw.reshape(4, 6, 5, 12, 3, 2)[0] == 0)  # halo plane stays zero
+    back = states.read_reference_dat(str(path), dims, 3)
+    np.testing.assert_array_equal(back, psi)
