@@ -11,6 +11,12 @@ in FP32 (FP64 reductions) and in FP64; the two final states are compared.
 `--workload scalar` runs the scalar staggered case instead: 192^3 BCC blocks, box 9.6^3,
 g = 300, vortex-ring imprint, dt = 1e-4.
 
+`--workload ensemble` runs 48 independent spin-2 runs (c0 = 1e4, c1 = +-200, c2 = +-50 with
+seeded signs) on mixed Cube/BCC/FCC/A15/C15 meshes of 32..96 blocks per axis (seeded), each
+with its own 8-knot Bz ramp U[-0.2, 0.2] and noisy ferromagnetic Thomas-Fermi state, dt = 1e-4,
+FP32 form: first one after another (each member stepped alone), then all together as one
+batched launch per sweep replayed from a captured hipGraph, from the same initial states.
+
 One step = one sweep over all unknowns (3-level) or one odd+even pair (staggered); a
 node-update = one unknown advanced by one sweep.  Prints one JSON line.
 """
@@ -85,6 +91,118 @@ def workload_scalar():
     info = dict(workload="scalar_bcc_192^3", lattice="bcc", cells=list(cells), box=box, components=1,
                 scheme="staggered", dt=cfg.dt, g=cfg.c0, imprint="vortex ring R=2")
     return cfg, None, info
+
+
+def ensemble_members(seed=4848, n=48, box=16.0):
+    """Seeded member list: (config, knots, seed) with c1, c2 signs, lattice and blocks per axis drawn."""
+    from spacetime_amd import gpe, lattice
+    rng = np.random.default_rng(seed)
+    names = rng.choice(list(lattice.LATTICES), n)
+    dims = rng.integers(32, 97, size=(n, 3))
+    signs = rng.choice([-1.0, 1.0], size=(n, 2))
+    knots = rng.uniform(-0.2, 0.2, size=(n, 8))
+    out = []
+    for i in range(n):
+        d = tuple(int(v) for v in dims[i])
+        # the smallest axis spans `box`; the finest block edge is box / 96, inside the
+        # 3-level leapfrog's stability bound dt * ||H|| < 1 for every lattice at dt = 1e-4
+        cfg = gpe.Config.centred_box(str(names[i]), d, box / min(d), ncomp=5, dt=1e-4, c0=1.0e4,
+                                     c1=200.0 * signs[i, 0], c2=50.0 * signs[i, 1])
+        out.append((cfg, knots[i], seed + 1 + i))
+    return out
+
+
+def run_ensemble(args, log):
+    """Aggregate node-updates/s of the batched ensemble vs the same runs one after another."""
+    from spacetime_amd import gpe, states
+    members = ensemble_members()
+    duration = 5000 * 1e-4
+    steppers = []
+    nodes = 0.0
+    for cfg, knots, _ in members:
+        st = gpe.Stepper(cfg)
+        st.set_ramp(b_segments=states.knot_ramp(knots, duration))
+        steppers.append(st)
+        nodes += float(np.prod(cfg.dims) * cfg.V)
+    gmu = 1.0e4
+    mu = 0.5 * (15.0 * gmu / (4.0 * np.pi)) ** 0.4  # Thomas-Fermi mu of a unit-norm isotropic trap
+    ferro = np.array([1, 0, 0, 0, 0], dtype=np.complex128)
+
+    def prepare():
+        for (cfg, _, sd), st in zip(members, steppers):
+            st.fill_thomas_fermi(gmu, mu, ferro, noise=0.005, seed=sd)
+            st.scale(1.0 / np.sqrt(st.observables(0).norm))
+            st.start(gpe.START_EULER)
+
+    log("ensemble: %d members, %.3e unknowns in total (%.1f GB of FP32 levels)" % (
+        len(members), nodes, nodes * 5 * 8 * 2 / 1e9))
+    prepare()
+    tune = []
+    t0 = time.perf_counter()
+    for st in steppers:
+        tune.append(st.autotune(args.tune_sweeps))
+    log("autotune: %d members in %.1fs" % (len(steppers), time.perf_counter() - t0))
+    prepare()
+
+    # one after another: each member alone, eager launches
+    for st in steppers:
+        st.step(args.warmup)
+        st.sync()
+    _sync()
+    t0 = time.perf_counter()
+    bytes_sweep = 0.0
+    for st in steppers:
+        st.step(args.steps)
+        st.sync()
+        bytes_sweep += st.last_step_timing()[1]
+    _sync()
+    wall_seq = time.perf_counter() - t0
+    obs_seq = [st.observables(0) for st in steppers]
+    log("sequential: %d steps in %.3fs, %.3e node-updates/s" % (args.steps, wall_seq, nodes * args.steps / wall_seq))
+
+    prepare()
+    G = args.graph_sweeps
+    with gpe.Ensemble(steppers) as ens:
+        ens.step(args.warmup, G if args.warmup % G == 0 else 0)
+        _sync()
+        t0 = time.perf_counter()
+        ens.step(args.steps, G)
+        _sync()
+        wall_ens = time.perf_counter() - t0
+        ms_sweep, _ = ens.timing()
+    obs_ens = [st.observables(0) for st in steppers]
+    rel_e = max(abs(a.energy - b.energy) / abs(b.energy) for a, b in zip(obs_ens, obs_seq))
+    rel_n = max(abs(a.norm - b.norm) / b.norm for a, b in zip(obs_ens, obs_seq))
+    drift = max(abs(o.norm - 1.0) for o in obs_ens)
+    finite = all(o.finite for o in obs_ens)
+    log("ensemble: %d steps in %.3fs, %.4f ms/sweep, %.3e node-updates/s; max rel energy diff vs sequential %.2e" % (
+        args.steps, wall_ens, ms_sweep, nodes * args.steps / wall_ens, rel_e))
+    for st in steppers:
+        st.close()
+    counts = {}
+    for cfg, _, _ in members:
+        counts[cfg.lattice] = counts.get(cfg.lattice, 0) + 1
+    achieved = bytes_sweep / (ms_sweep * 1e-3) / 1e9
+    return {
+        "metric": "aggregate node-updates/s of a 48-run spin-2 ensemble (one batched launch per sweep, hipGraph) "
+                  "vs the runs one after another",
+        "value": nodes * args.steps / wall_ens,
+        "unit": "node-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall_ens / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "ensemble_spin2_48_mixed", "members": len(members), "lattices": counts,
+                   "blocks_per_axis": [32, 96], "unknowns": nodes, "components": 5, "dt": 1e-4,
+                   "graph_sweeps": G, "autotuned": [[t["tile_rows"], t["xcd_remap"]] for t in tune]},
+        "sequential": {"value": nodes * args.steps / wall_seq, "ms_per_step": 1e3 * wall_seq / args.steps},
+        "speedup_vs_sequential": wall_seq / wall_ens,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel_ms_per_sweep": ms_sweep,
+                     "algorithmic_bytes_per_sweep": bytes_sweep},
+        "accuracy": {"max_rel_energy_diff_vs_sequential": rel_e, "max_rel_norm_diff_vs_sequential": rel_n,
+                     "max_norm_drift": drift, "finite": finite},
+        "cpu_baseline": None,
+    }
 
 
 def prepare_state(cfg, knots, itp_steps, itp_dt, log):
@@ -184,9 +302,12 @@ def cpu_baseline(cfg, psi0, log, budget_s=15.0):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: 20000 spin1, 10000 scalar, 5000 ensemble)")
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", choices=["spin1", "scalar"], default="spin1")
+    ap.add_argument("--workload", choices=["spin1", "scalar", "ensemble"], default="spin1")
+    ap.add_argument("--graph-sweeps", type=int, default=20)
+    ap.add_argument("--tune-sweeps", type=int, default=6)
     ap.add_argument("--itp-steps", type=int, default=1000)
     ap.add_argument("--itp-dt", type=float, default=2e-4)
     ap.add_argument("--no-cpu", action="store_true")
@@ -198,6 +319,15 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     log = (lambda *a: None) if args.quiet else (lambda m: print("[bench] " + m, file=sys.stderr, flush=True))
+    if args.steps is None:
+        args.steps = {"spin1": 20000, "scalar": 10000, "ensemble": 5000}[args.workload]
+    if args.workload == "ensemble" and (args.graph_sweeps <= 0 or args.graph_sweeps % 2 or
+                                        args.steps % args.graph_sweeps):
+        ap.error("--graph-sweeps must be even and divide --steps")
+
+    if args.workload == "ensemble":
+        print(json.dumps(run_ensemble(args, log)))
+        return
 
     from spacetime_amd import gpe
     import dataclasses

@@ -204,9 +204,10 @@ int gpe_vortex_relax(const gpe_vortex_params *vp, double *f, double *mu);
  * every unknown of cfg's grid into a host ABI array (component 0 of ncomp; the others 0). */
 int gpe_vortex_fill(const gpe_config *cfg, const gpe_vortex_params *vp, const double *f, double *psi);
 
-/* Fill level 0 on the device: psi_c = w_c sqrt(max(mu - V, 0) / g) (1 + noise * eps_c), eps_c
+/* Fill level 0 on the device: psi_c = sqrt(max(mu - V, 0) / g) (w_c + noise * eps_c), eps_c
  * uniform in the unit disk from a counter-based hash of (seed, unknown, component), so the
- * result does not depend on the launch.  weights: ncomp complex values (re, im).  Sets the
+ * result does not depend on the launch (w = 1: multiplicative noise; a unit spinor w: spinor
+ * noise of relative size `noise`).  weights: ncomp complex values (re, im).  Sets the
  * run time to t like gpe_set_state.  For large grids where host-side states do not pay. */
 int gpe_fill_thomas_fermi(gpe_ctx *ctx, double g, double mu, const double *weights, double noise,
                           unsigned long long seed, double t);

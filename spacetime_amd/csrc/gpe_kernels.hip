@@ -417,7 +417,8 @@ __device__ __forceinline__ double hash_uniform(unsigned long long seed, unsigned
     return (double)(z >> 11) * (1.0 / 9007199254740992.0);
 }
 
-// psi_c = w_c sqrt(max(mu - V, 0) / g) (1 + amp * eps_c), eps_c uniform in the unit disk.
+// psi_c = sqrt(max(mu - V, 0) / g) (w_c + amp * eps_c), eps_c uniform in the unit disk: multiplicative
+// noise for one component with w = 1, spinor noise around the spinor w otherwise.
 template <typename R>
 __global__ __launch_bounds__(256) void k_fill_tf(Geometry g, Tables tb, Physics ph, int ncomp, FillArgs f, void *psi)
 {
@@ -447,10 +448,9 @@ __global__ __launch_bounds__(256) void k_fill_tf(Geometry g, Tables tb, Physics 
             er = rad * cos(ang);
             ei = rad * sin(ang);
         }
-        const double br = a * (1.0 + f.amp * er), bi = a * f.amp * ei;  // a (1 + amp eps)
-        vec2_t<R> o;
-        o.x = (R)(f.w[2 * c] * br - f.w[2 * c + 1] * bi);
-        o.y = (R)(f.w[2 * c] * bi + f.w[2 * c + 1] * br);
+        vec2_t<R> o;  // a (w_c + amp eps_c)
+        o.x = (R)(a * (f.w[2 * c] + f.amp * er));
+        o.y = (R)(a * (f.w[2 * c + 1] + f.amp * ei));
         out[line_index(g, c, z, y, v) + x] = o;
     }
 }

@@ -244,7 +244,7 @@ class Stepper:
         self._check(self.lib.gpe_get_state_device(self._h, level, ctypes.c_void_p(ptr)), "gpe_get_state_device")
 
     def fill_thomas_fermi(self, g, mu, weights=None, noise=0.0, seed=0, t=0.0):
-        """Level 0 := w_c sqrt(max(mu - V, 0) / g) (1 + noise eps_c) on the device
+        """Level 0 := sqrt(max(mu - V, 0) / g) (w_c + noise eps_c) on the device
         (gpe_fill_thomas_fermi); weights default to all weight in component 0."""
         C = self.config.ncomp
         w = np.zeros(C, dtype=np.complex128)
