@@ -12,7 +12,7 @@ in FP32 (FP64 reductions) and in FP64; the two final states are compared.
 g = 300, vortex-ring imprint, dt = 1e-4.
 
 `--workload ensemble` runs 48 independent spin-2 runs (c0 = 1e4, c1 = +-200, c2 = +-50 with
-seeded signs) on mixed Cube/BCC/FCC/A15/C15 meshes of 32..96 blocks per axis (seeded), each
+seeded signs) on mixed Cube/BCC/FCC/A15/C15 meshes of n^3 blocks, n in 32..96 (seeded), each
 with its own 8-knot Bz ramp U[-0.2, 0.2] and noisy ferromagnetic Thomas-Fermi state, dt = 1e-4,
 FP32 form: first one after another (each member stepped alone), then all together as one
 batched launch per sweep replayed from a captured hipGraph, from the same initial states.
@@ -98,15 +98,15 @@ def ensemble_members(seed=4848, n=48, box=16.0):
     from spacetime_amd import gpe, lattice
     rng = np.random.default_rng(seed)
     names = rng.choice(list(lattice.LATTICES), n)
-    dims = rng.integers(32, 97, size=(n, 3))
+    dims = rng.integers(32, 97, size=n)
     signs = rng.choice([-1.0, 1.0], size=(n, 2))
     knots = rng.uniform(-0.2, 0.2, size=(n, 8))
     out = []
     for i in range(n):
-        d = tuple(int(v) for v in dims[i])
-        # the smallest axis spans `box`; the finest block edge is box / 96, inside the
-        # 3-level leapfrog's stability bound dt * ||H|| < 1 for every lattice at dt = 1e-4
-        cfg = gpe.Config.centred_box(str(names[i]), d, box / min(d), ncomp=5, dt=1e-4, c0=1.0e4,
+        d = (int(dims[i]),) * 3
+        # box^3 with n^3 blocks; the finest block edge, box / 96, keeps dt * ||H|| < 1 (the
+        # 3-level leapfrog's stability bound) for every lattice at dt = 1e-4
+        cfg = gpe.Config.centred_box(str(names[i]), d, box / d[0], ncomp=5, dt=1e-4, c0=1.0e4,
                                      c1=200.0 * signs[i, 0], c2=50.0 * signs[i, 1])
         out.append((cfg, knots[i], seed + 1 + i))
     return out
