@@ -103,6 +103,10 @@ typedef struct {
     double max_density;    /* max |psi|^2                                       */
     double time;           /* simulation time of this level                    */
     int finite;            /* 0 if any non-finite value was seen                */
+    double spin;           /* sum dv |psi^dag F psi| (Spin1GpeBCC.cu:235-271,992-1020; the
+                              reference's local spin z part is not divided by n, :258-261) */
+    double moment[3];      /* sum dv |psi|^2 r: the centre of mass of a unit-norm state
+                              (GPUGrossPitaevskii_BCC.cu:349-413, which copies back only x) */
 } gpe_observables_t;
 
 /* Create a context: allocates both time levels on the current HIP device and uploads
@@ -144,6 +148,21 @@ int gpe_itp(gpe_ctx *ctx, int nsteps, double dt_itp);
 /* FP64 observables of a level.  Replaces density/innerProduct/localAvgSpinAndDensity and
  * the integrate trees (Spin1GpeBCC.cu:191-271,328-392,955-1020). */
 int gpe_observables(gpe_ctx *ctx, int level, gpe_observables_t *out);
+
+/* Per-unknown field maps of a level, copied to `out` in ABI order (z, y, x, v) with
+ * gpe_field_width(kind) doubles per unknown.  Replaces localAvgSpinAndDensity and uvTheta
+ * (Spin1GpeBCC.cu:235-326), which feed the reference's VTK/state writers.
+ *   GPE_FIELD_DENSITY    |psi|^2
+ *   GPE_FIELD_SPIN       psi^dag F psi / |psi|^2 (0 where the density is 0)
+ *   GPE_FIELD_SPIN_NORM  |psi^dag F psi|
+ *   GPE_FIELD_UV_THETA   spin-1 only: u (3), v (3), theta with a e^{i theta} = u + i v, u.v = 0,
+ *                        |u| >= |v| (else swapped), theta in [0, pi)                        */
+#define GPE_FIELD_DENSITY 0
+#define GPE_FIELD_SPIN 1
+#define GPE_FIELD_SPIN_NORM 2
+#define GPE_FIELD_UV_THETA 3
+int gpe_field_width(int kind);
+int gpe_local_fields(gpe_ctx *ctx, int level, int kind, double *out);
 
 /* Scale a level by a real factor (normalize, Spin1GpeBCC.cu:412-434). */
 int gpe_scale(gpe_ctx *ctx, int level, double factor);

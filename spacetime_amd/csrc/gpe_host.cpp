@@ -615,7 +615,41 @@ int gpe_observables(gpe_ctx *c, int level, gpe_observables_t *out)
     out->max_density = h[OBS_MAXDENS];
     out->time = t;
     out->finite = (h[OBS_NONFINITE] == 0.0 && std::isfinite(h[OBS_NORM]) && std::isfinite(h[OBS_ENERGY])) ? 1 : 0;
+    out->spin = h[OBS_SPIN];
+    out->moment[0] = h[OBS_RX];
+    out->moment[1] = h[OBS_RY];
+    out->moment[2] = h[OBS_RZ];
     return out->finite ? GPE_OK : fail(c, GPE_ERANGE, "non-finite value in the state");
+}
+
+int gpe_field_width(int kind)
+{
+    switch (kind) {
+    case GPE_FIELD_DENSITY: return 1;
+    case GPE_FIELD_SPIN: return 3;
+    case GPE_FIELD_SPIN_NORM: return 1;
+    case GPE_FIELD_UV_THETA: return 7;
+    default: return GPE_EINVAL;
+    }
+}
+
+int gpe_local_fields(gpe_ctx *c, int level, int kind, double *out)
+{
+    if (!c) return GPE_EINVAL;
+    const int width = gpe_field_width(kind);
+    if (width < 0) return fail(c, GPE_EINVAL, "unknown field kind");
+    if (!out) return fail(c, GPE_EINVAL, "null output");
+    if (kind == GPE_FIELD_UV_THETA && c->cfg.ncomp != 3) return fail(c, GPE_EINVAL, "u, v, theta need spin-1");
+    int rc = check_level(c, level);
+    if (rc) return rc;
+    const size_t bytes = sizeof(double) * (size_t)width * (size_t)c->cfg.nx * c->cfg.ny * c->cfg.nz * c->g.V;
+    double *d = nullptr;
+    if (hipMalloc((void **)&d, bytes) != hipSuccess) return fail(c, GPE_ENOMEM, "field map does not fit");
+    hipError_t e = launch_local_fields(c->fp64, c->cfg.ncomp, c->g, c->level[level], kind, d, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    return e == hipSuccess ? GPE_OK : hip_fail(c, e, "local fields");
 }
 
 int gpe_scale(gpe_ctx *c, int level, double factor)

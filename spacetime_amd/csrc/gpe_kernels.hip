@@ -319,6 +319,10 @@ __global__ __launch_bounds__(256) void k_observables(ObsArgs a)
         acc[OBS_MZ] = Fz * dv;
         acc[OBS_MAXDENS] = n;
         acc[OBS_NONFINITE] = (isfinite(n) && isfinite(e)) ? 0.0 : 1.0;
+        acc[OBS_SPIN] = sqrt(Fx * Fx + Fy * Fy + Fz * Fz) * dv;
+        acc[OBS_RX] = n * rx * dv;
+        acc[OBS_RY] = n * ry * dv;
+        acc[OBS_RZ] = n * rz * dv;
     }
     __shared__ double red[4][OBS_COUNT];
 #pragma unroll
@@ -464,6 +468,98 @@ hipError_t launch_fill_tf(int fp64, const Geometry &g, const Tables &tb, const P
         hipLaunchKernelGGL(k_fill_tf<double>, dim3(grid), dim3(256), 0, s, g, tb, ph, ncomp, f, psi);
     else
         hipLaunchKernelGGL(k_fill_tf<float>, dim3(grid), dim3(256), 0, s, g, tb, ph, ncomp, f, psi);
+    return hipGetLastError();
+}
+
+// Per-unknown field maps in ABI order (z, y, x, v), `width` doubles each.
+//   FIELD_DENSITY    n = sum_m |psi_m|^2
+//   FIELD_SPIN       <F> = psi^dag F psi / n (0 where n = 0)
+//   FIELD_SPIN_NORM  |psi^dag F psi|
+//   FIELD_UV_THETA   spin-1 Cartesian vector a = (psi_-1 - psi_1, -i (psi_-1 + psi_1), sqrt2 psi_0) / sqrt2
+//                    = m + i n; theta = atan2(-2 m.n, |m|^2 - |n|^2) / 2 in [0, pi) makes u = Re(a e^{i theta}),
+//                    v = Im(a e^{i theta}) orthogonal; (u, v) ordered so that |u| >= |v|; out = u, v, theta.
+template <typename R, int C>
+__global__ __launch_bounds__(256) void k_local_fields(Geometry g, const vec2_t<R> *psi, int kind, double *out)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long total = (long long)g.nz * g.ny * g.nx * g.V;
+    if (i >= total) return;
+    long long r = i;
+    const int v = (int)(r % g.V);
+    r /= g.V;
+    const int x = (int)(r % g.nx);
+    r /= g.nx;
+    const int y = (int)(r % g.ny);
+    const int z = (int)(r / g.ny);
+    double pr[C], pi[C];
+    double n = 0;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const vec2_t<R> o = psi[line_index(g, c, z, y, v) + x];
+        pr[c] = o.x;
+        pi[c] = o.y;
+        n += pr[c] * pr[c] + pi[c] * pi[c];
+    }
+    if (kind == FIELD_DENSITY) {
+        out[i] = n;
+        return;
+    }
+    if (kind == FIELD_UV_THETA) {
+        if constexpr (C == 3) {
+            const double s = 0.7071067811865476;
+            const double m[3] = {s * (pr[2] - pr[0]), s * (pi[2] + pi[0]), pr[1]};
+            const double q[3] = {s * (pi[2] - pi[0]), -s * (pr[2] + pr[0]), pi[1]};
+            const double mq = m[0] * q[0] + m[1] * q[1] + m[2] * q[2];
+            const double mm = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
+            const double qq = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+            double th = 0.5 * atan2(-2.0 * mq, mm - qq);
+            if (th < 0) th += 3.141592653589793;
+            const double ct = cos(th), st = sin(th);
+            double u[3], w[3], uu = 0, ww = 0;
+            for (int k = 0; k < 3; ++k) {
+                u[k] = m[k] * ct - q[k] * st;
+                w[k] = m[k] * st + q[k] * ct;
+                uu += u[k] * u[k];
+                ww += w[k] * w[k];
+            }
+            double *o = out + 7 * i;
+            const bool keep = uu >= ww;
+            for (int k = 0; k < 3; ++k) {
+                o[k] = keep ? u[k] : w[k];
+                o[3 + k] = keep ? w[k] : u[k];
+            }
+            o[6] = th;
+        }
+        return;
+    }
+    double F[3] = {0, 0, 0};
+    if constexpr (C > 1) {
+        double tr[C], ti[C];
+        for (int a = 0; a < 3; ++a) {
+            spin_dot<double, C>(a == 0, a == 1, a == 2, pr, pi, tr, ti);
+            for (int c = 0; c < C; ++c) F[a] += pr[c] * tr[c] + pi[c] * ti[c];
+        }
+    }
+    if (kind == FIELD_SPIN_NORM) {
+        out[i] = sqrt(F[0] * F[0] + F[1] * F[1] + F[2] * F[2]);
+    } else {
+        const double inv = n > 0 ? 1.0 / n : 0.0;
+        for (int a = 0; a < 3; ++a) out[3 * i + a] = F[a] * inv;
+    }
+}
+
+hipError_t launch_local_fields(int fp64, int ncomp, const Geometry &g, const void *psi, int kind, double *out,
+                               hipStream_t s)
+{
+    const long long total = (long long)g.nz * g.ny * g.nx * g.V;
+    const unsigned grid = (unsigned)((total + 255) / 256);
+#define LF(R, C) hipLaunchKernelGGL((k_local_fields<R, C>), dim3(grid), dim3(256), 0, s, g, (const vec2_t<R> *)psi, kind, out)
+    if (fp64) {
+        if (ncomp == 1) LF(double, 1); else if (ncomp == 3) LF(double, 3); else LF(double, 5);
+    } else {
+        if (ncomp == 1) LF(float, 1); else if (ncomp == 3) LF(float, 3); else LF(float, 5);
+    }
+#undef LF
     return hipGetLastError();
 }
 

@@ -24,6 +24,7 @@ FP32, FP64 = 0, 1
 LEAPFROG3, STAGGERED = 0, 1
 START_EULER, START_PHASE, START_COPY = 0, 1, 2
 OBJ_OVERLAP, OBJ_ENERGY, OBJ_MAGNETIZATION_Z = 0, 1, 2
+FIELD_DENSITY, FIELD_SPIN, FIELD_SPIN_NORM, FIELD_UV_THETA = 0, 1, 2, 3
 ERRORS = {-22: "EINVAL", -12: "ENOMEM", -5: "EDEVICE", -71: "ESTATE", -34: "ERANGE"}
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -55,7 +56,7 @@ class _Obs(ctypes.Structure):
     _fields_ = [
         ("norm", ctypes.c_double), ("energy", ctypes.c_double), ("expect_h", ctypes.c_double),
         ("magnetization", ctypes.c_double * 3), ("max_density", ctypes.c_double), ("time", ctypes.c_double),
-        ("finite", ctypes.c_int),
+        ("finite", ctypes.c_int), ("spin", ctypes.c_double), ("moment", ctypes.c_double * 3),
     ]
 
 
@@ -117,6 +118,8 @@ def load_library(path=LIB_PATH):
     lib.gpe_ensemble_timing.argtypes = [vp, _dp, _dp]
     lib.gpe_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _dp]
     lib.gpe_set_trap.argtypes = [vp, _dp]
+    lib.gpe_field_width.argtypes = [ctypes.c_int]
+    lib.gpe_local_fields.argtypes = [vp, ctypes.c_int, ctypes.c_int, _dp]
     lib.gpe_fill_thomas_fermi.argtypes = [vp, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_double,
                                           ctypes.c_ulonglong, ctypes.c_double]
     _LIB = lib
@@ -186,6 +189,8 @@ class Observables:
     max_density: float
     time: float
     finite: bool
+    spin: float = 0.0
+    moment: np.ndarray = None
 
 
 class Stepper:
@@ -335,7 +340,17 @@ class Stepper:
         if check or rc not in (0, -34):
             self._check(rc, "gpe_observables")
         return Observables(o.norm, o.energy, o.expect_h, np.array(o.magnetization[:]), o.max_density, o.time,
-                           bool(o.finite))
+                           bool(o.finite), o.spin, np.array(o.moment[:]))
+
+    def local_fields(self, kind, level=0):
+        """Per-unknown field map (gpe_local_fields), shaped (nz, ny, nx, V, width)."""
+        w = self.lib.gpe_field_width(kind)
+        if w < 0:
+            raise ValueError("unknown field kind %r" % kind)
+        nx, ny, nz = self.config.dims
+        out = np.empty((nz, ny, nx, self.config.V, w), dtype=np.float64)
+        self._check(self.lib.gpe_local_fields(self._h, level, kind, out.ctypes.data_as(_dp)), "gpe_local_fields")
+        return out
 
     def time(self):
         t = ctypes.c_double()

@@ -47,6 +47,11 @@ def test_lattice_values_host_only(lib):
     assert lib.gpe_lattice_values(7) == -22
 
 
+def test_field_width_host_only(lib):
+    assert [lib.gpe_field_width(k) for k in range(4)] == [1, 3, 1, 7]
+    assert lib.gpe_field_width(9) == -22
+
+
 def test_create_rejects_bad_config_without_device(lib):
     from spacetime_amd import gpe
     cfg = gpe.Config(lattice="bcc", dims=(0, 4, 4)).to_c()

@@ -1,0 +1,92 @@
+"""Per-unknown field maps (gpe_local_fields) and the spin / first-moment observables against
+textbook spin algebra on random states (dense F matrices from the angular-momentum ladder),
+plus closed forms of the spin-1 (u, v, theta) decomposition on polar and ferromagnetic
+spinors.  FP64 contexts: 1e-12 relative."""
+import numpy as np
+import pytest
+
+from helpers import random_state
+
+pytestmark = pytest.mark.gpu
+
+gpe = pytest.importorskip("spacetime_amd.gpe")
+
+
+def _spin_matrices(F):
+    m = np.arange(F, -F - 1, -1).astype(float)
+    n = len(m)
+    fp = np.zeros((n, n))
+    for i in range(n - 1):
+        mm = m[i + 1]
+        fp[i, i + 1] = np.sqrt(F * (F + 1) - mm * (mm + 1))
+    return [0.5 * (fp + fp.T) + 0j, -0.5j * (fp - fp.T), np.diag(m) + 0j]
+
+
+def _cfg(ncomp, lattice="bcc", dims=(5, 4, 3)):
+    return gpe.Config.centred_box(lattice, dims, 0.5, ncomp=ncomp, precision=gpe.FP64, dt=1e-4,
+                                  trap=(1.0, 1.1, 0.9), c0=50.0, c1=5.0, c2=-3.0)
+
+
+@pytest.mark.parametrize("ncomp,lattice", [(3, "bcc"), (5, "fcc"), (1, "a15")])
+def test_density_spin_maps_and_integrals(ncomp, lattice):
+    from spacetime_amd import states
+    cfg = _cfg(ncomp, lattice)
+    psi = random_state(cfg.shape, 11)
+    psi[:, 0, 0, 0, 0] = 0.0  # one empty unknown: <F> defined as 0 there
+    with gpe.Stepper(cfg) as st:
+        st.set_state(psi)
+        dens = st.local_fields(gpe.FIELD_DENSITY)[..., 0]
+        spin = st.local_fields(gpe.FIELD_SPIN)
+        snorm = st.local_fields(gpe.FIELD_SPIN_NORM)[..., 0]
+        obs = st.observables(0)
+        if ncomp != 3:
+            with pytest.raises(gpe.GpeError, match="EINVAL"):
+                st.local_fields(gpe.FIELD_UV_THETA)
+    n = np.sum(np.abs(psi) ** 2, axis=0)
+    np.testing.assert_allclose(dens, n, rtol=1e-13, atol=0)
+    if ncomp > 1:
+        f = _spin_matrices((ncomp - 1) // 2)
+        Fv = np.stack([np.einsum("czyxv,cd,dzyxv->zyxv", psi.conj(), fa, psi).real for fa in f], axis=-1)
+    else:
+        Fv = np.zeros(n.shape + (3,))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        expect = np.where(n[..., None] > 0, Fv / n[..., None], 0.0)
+    np.testing.assert_allclose(spin, expect, rtol=0, atol=1e-12 * max(1.0, np.abs(expect).max()))
+    assert np.all(spin[0, 0, 0, 0] == 0.0)
+    np.testing.assert_allclose(snorm, np.linalg.norm(Fv, axis=-1), rtol=0, atol=1e-12 * n.max())
+    dv = states.dual_volumes(cfg)
+    r = states.positions(cfg)
+    assert obs.spin == pytest.approx(np.sum(np.linalg.norm(Fv, axis=-1) * dv), rel=1e-12, abs=1e-14)
+    np.testing.assert_allclose(obs.moment, np.sum((n * dv)[..., None] * r, axis=(0, 1, 2, 3)), rtol=1e-12,
+                               atol=1e-12 * np.sum(n * dv))
+    np.testing.assert_allclose(obs.magnetization, np.sum(Fv * dv[:, None], axis=(0, 1, 2, 3)), rtol=1e-12,
+                               atol=1e-12 * np.sum(n * dv))
+
+
+def test_uv_theta_spin1():
+    cfg = _cfg(3)
+    psi = random_state(cfg.shape, 12)
+    psi[:, 0, 0, 0, 0] = (0.0, 0.7, 0.0)          # polar: a = (0, 0, 0.7), theta = 0
+    psi[:, 0, 0, 0, 1] = (0.6, 0.0, 0.0)          # ferro +1: |m| = |n|, m.n = 0, theta = 0
+    with gpe.Stepper(cfg) as st:
+        st.set_state(psi)
+        uvt = st.local_fields(gpe.FIELD_UV_THETA)
+    u, v, th = uvt[..., 0:3], uvt[..., 3:6], uvt[..., 6]
+    s = 1 / np.sqrt(2)
+    a = np.stack([s * (psi[2] - psi[0]), -1j * s * (psi[2] + psi[0]), psi[1]], axis=-1)
+    # closed forms
+    np.testing.assert_allclose(uvt[0, 0, 0, 0], [0, 0, 0.7, 0, 0, 0, 0], atol=1e-15)
+    np.testing.assert_allclose(uvt[0, 0, 0, 1], [-0.6 * s, 0, 0, 0, -0.6 * s, 0, 0], atol=1e-15)
+    # properties: theta in [0, pi), u . v = 0, |u| >= |v|, {u + i v, v + i u...} from a e^{i theta}
+    assert np.all((th >= 0) & (th < np.pi))
+    np.testing.assert_allclose(np.sum(u * v, axis=-1), 0.0, atol=1e-12)
+    assert np.all(np.linalg.norm(u, axis=-1) >= np.linalg.norm(v, axis=-1) - 1e-14)
+    rot = a * np.exp(1j * th)[..., None]
+    keep = np.linalg.norm(rot.real, axis=-1) >= np.linalg.norm(rot.imag, axis=-1)
+    np.testing.assert_allclose(np.where(keep[..., None], rot.real, rot.imag), u, atol=1e-12)
+    np.testing.assert_allclose(np.where(keep[..., None], rot.imag, rot.real), v, atol=1e-12)
+    # the branch: tan(2 theta) = -2 m.n / (|m|^2 - |n|^2) with the sign of atan2
+    m, q = a.real, a.imag
+    th_ref = 0.5 * np.arctan2(-2 * np.sum(m * q, -1), np.sum(m * m, -1) - np.sum(q * q, -1))
+    th_ref = np.where(th_ref < 0, th_ref + np.pi, th_ref)
+    np.testing.assert_allclose(th, th_ref, atol=1e-12)
