@@ -11,6 +11,11 @@ in FP32 (FP64 reductions) and in FP64; the two final states are compared.
 `--workload scalar` runs the scalar staggered case instead: 192^3 BCC blocks, box 9.6^3,
 g = 300, vortex-ring imprint, dt = 1e-4.
 
+`--workload adjoint` runs the discrete adjoint of the spin-1 physics on 64^3 BCC blocks: 4000
+sweeps, checkpoints every 200, gradients of the overlap with a target state (made by a seeded
+reference ramp) with respect to the 16 Bz knots and 3 trap frequencies, FP32 and FP64, checked
+against central finite differences of FP64 forward runs (h = 1e-4).
+
 `--workload ensemble` runs 48 independent spin-2 runs (c0 = 1e4, c1 = +-200, c2 = +-50 with
 seeded signs) on mixed Cube/BCC/FCC/A15/C15 meshes of n^3 blocks, n in 32..96 (seeded), each
 with its own 8-knot Bz ramp U[-0.2, 0.2] and noisy ferromagnetic Thomas-Fermi state, dt = 1e-4,
@@ -201,7 +206,8 @@ def run_ensemble(args, log):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": "ensemble_spin2_48_mixed", "members": len(members), "lattices": counts,
                    "blocks_per_axis": [32, 96], "unknowns": nodes, "components": 5, "dt": 1e-4,
-                   "graph_sweeps": G, "autotuned": [[t["tile_rows"], t["xcd_remap"]] for t in tune]},
+                   "graph_sweeps": G,
+                   "autotuned": [[t["tile_rows"], t["xcd_remap"], t["x_per_lane"]] for t in tune]},
         "sequential": {"value": nodes * args.steps / wall_seq, "ms_per_step": 1e3 * wall_seq / args.steps},
         "speedup_vs_sequential": wall_seq / wall_ens,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
@@ -210,6 +216,90 @@ def run_ensemble(args, log):
         "accuracy": {"max_rel_energy_diff_vs_sequential": rel_e, "max_rel_norm_diff_vs_sequential": rel_n,
                      "max_norm_drift": drift, "finite": finite},
         "cpu_baseline": None,
+    }
+
+
+def run_adjoint(args, log):
+    """Spin-1 BCC 64^3 discrete adjoint: gradients of |<target|psi(T)>|^2 with respect to the 16
+    Bz knots and the 3 trap frequencies, FP32 sweeps with FP64 accumulation, checkpoints every
+    200 sweeps; checked against central finite differences of FP64 forward runs (h = 1e-4)."""
+    from spacetime_amd import gpe, states
+    from spacetime_amd.adjoint import RampTrapProblem
+    import dataclasses
+    cells = (64, 64, 64)
+    box = 24.0
+    cfg = gpe.Config.centred_box("bcc", cells, box / cells[0], ncomp=3, dt=1e-4, trap=(1.0, 1.0, 166.0 / 126.0),
+                                 c0=1.0e4, c1=500.0, q=0.1)
+    cfg64 = dataclasses.replace(cfg, precision=gpe.FP64)
+    nsweeps, K = args.steps, 200
+    duration = nsweeps * cfg.dt
+    knots = np.random.default_rng(2024).uniform(-0.1, 0.1, 16)
+    ref_knots = np.random.default_rng(4000).uniform(-0.1, 0.1, 16)
+    psi0 = prepare_state(cfg, knots, args.itp_steps, args.itp_dt, log)
+    with gpe.Stepper(cfg64) as st:  # the target: the same start under the seeded reference ramp
+        st.set_ramp(q_segments=[(4e-3, 1e30, 0)], b_segments=states.knot_ramp(ref_knots, duration))
+        st.set_state(psi0)
+        st.start(gpe.START_EULER)
+        st.step(nsweeps - 1)
+        target = st.get_state(0)
+    out = {}
+    for name, c in (("fp32", cfg), ("fp64", cfg64)):
+        prob = RampTrapProblem(c, psi0, nsweeps, K, duration, objective=gpe.OBJ_OVERLAP, target=target, bq=4e-3)
+        prob.value_and_grad(knots, c.trap)  # warm-up (allocations, autotune-free)
+        t0 = time.perf_counter()
+        J, gk, gt = prob.value_and_grad(knots, c.trap)
+        wall = time.perf_counter() - t0
+        res = prob.last
+        prob.close()
+        out[name] = dict(J=J, grad_knots=gk, grad_trap=gt, wall_s=wall, forward_ms=res["forward_ms"],
+                         backward_ms=res["backward_ms"], checkpoint_bytes=res["checkpoint_bytes"],
+                         segment_bytes=res["segment_bytes"])
+        log("adjoint %s: J=%.9f, %.3f s forward+backward (forward %.1f ms, backward %.1f ms), checkpoints %.2f GB, "
+            "segment %.2f GB" % (name, J, wall, res["forward_ms"], res["backward_ms"], res["checkpoint_bytes"] / 1e9,
+                                  res["segment_bytes"] / 1e9))
+    h = 1e-4
+    fd = RampTrapProblem(cfg64, psi0, nsweeps, K, duration, objective=gpe.OBJ_OVERLAP, target=target, bq=4e-3)
+    t0 = time.perf_counter()
+    gk_fd = np.zeros(16)
+    for j in range(16):
+        kp, km = knots.copy(), knots.copy()
+        kp[j] += h
+        km[j] -= h
+        gk_fd[j] = (fd.value(kp, cfg.trap) - fd.value(km, cfg.trap)) / (2 * h)
+    gt_fd = np.zeros(3)
+    for a in range(3):
+        tp, tm = np.array(cfg.trap, float), np.array(cfg.trap, float)
+        tp[a] += h
+        tm[a] -= h
+        gt_fd[a] = (fd.value(knots, tp) - fd.value(knots, tm)) / (2 * h)
+    fd.close()
+    log("finite differences: 38 FP64 forward runs in %.1f s" % (time.perf_counter() - t0))
+    g_fd = np.concatenate([gk_fd, gt_fd])
+    scale = float(np.abs(g_fd).max())
+    err = {name: float(np.abs(np.concatenate([r["grad_knots"], r["grad_trap"]]) - g_fd).max() / scale)
+           for name, r in out.items()}
+    log("max |adjoint - FD| / max |FD|: fp32 %.2e, fp64 %.2e" % (err["fp32"], err["fp64"]))
+    r32 = out["fp32"]
+    level_bytes = float(np.prod(cfg.dims) * cfg.V * 3 * 8)
+    return {
+        "metric": "spin-1 BCC 64^3 discrete adjoint: seconds per forward+backward (4000 sweeps, checkpoints every "
+                  "200), gradient error vs central finite differences",
+        "value": r32["wall_s"],
+        "unit": "s per forward+backward",
+        "n_gpus": 1, "steps": nsweeps, "warmup": 1, "ms_per_step": 1e3 * r32["wall_s"] / nsweeps,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "adjoint_spin1_bcc_64^3", "cells": list(cells), "box": box, "sweeps": nsweeps,
+                   "checkpoint_every": K, "parameters": "16 Bz knots + 3 trap frequencies", "objective": "overlap",
+                   "fd_step": h},
+        "memory_bytes": {"levels": 2 * level_bytes, "checkpoints": r32["checkpoint_bytes"],
+                         "segment": r32["segment_bytes"]},
+        "forward_ms": r32["forward_ms"], "backward_ms": r32["backward_ms"],
+        "fp64_form": {"value": out["fp64"]["wall_s"], "forward_ms": out["fp64"]["forward_ms"],
+                      "backward_ms": out["fp64"]["backward_ms"]},
+        "accuracy": {"max_rel_grad_err_fp32_vs_fd": err["fp32"], "max_rel_grad_err_fp64_vs_fd": err["fp64"],
+                     "J_fp32": r32["J"], "J_fp64": out["fp64"]["J"],
+                     "grad_fd": g_fd.tolist(), "grad_fp32": np.concatenate([r32["grad_knots"], r32["grad_trap"]]).tolist()},
+        "roofline": None, "cpu_baseline": None,
     }
 
 
@@ -223,6 +313,12 @@ def prepare_state(cfg, knots, itp_steps, itp_dt, log):
     psi[cfg.ncomp // 2] = tf
     del tf
     cfg64 = dataclasses.replace(cfg, precision=gpe.FP64, scheme=gpe.LEAPFROG3)
+    # imaginary-time Euler contracts every mode only for dt * lambda_max < 2: keep dt * ||H|| <= 1
+    from spacetime_amd import lattice
+    lam = lattice.get(cfg.lattice).kinetic_bound(cfg.block_scale) + mu
+    if itp_dt * lam > 1.0:
+        log("itp: dt %g lowered to %g (kinetic bound %.4g)" % (itp_dt, 1.0 / lam, lam))
+        itp_dt = 1.0 / lam
     t0 = time.time()
     with gpe.Stepper(cfg64) as st:
         st.set_state(psi)
@@ -240,7 +336,7 @@ def prepare_state(cfg, knots, itp_steps, itp_dt, log):
     return psi
 
 
-def run_form(cfg, psi0, knots, warmup, steps, log):
+def run_form(cfg, psi0, knots, warmup, steps, log, tune_sweeps=0):
     from spacetime_amd import gpe, states
     with gpe.Stepper(cfg) as st:
         if knots is not None:
@@ -248,6 +344,8 @@ def run_form(cfg, psi0, knots, warmup, steps, log):
         st.set_state(psi0)
         o0 = st.observables(0)
         st.start(gpe.START_EULER)
+        tune = st.autotune(tune_sweeps) if tune_sweeps else st.traversal()
+        log("%s traversal: %s" % ("fp64" if cfg.precision else "fp32", tune))
         if warmup:
             st.step(warmup)
         st.sync()
@@ -267,7 +365,7 @@ def run_form(cfg, psi0, knots, warmup, steps, log):
                kernel_node_updates_per_s=nodes / (ms_sweep * 1e-3),
                achieved_gbs=bytes_sweep / (ms_sweep * 1e-3) / 1e9,
                norm_drift=abs(o1.norm - o0.norm) / o0.norm, energy_drift=abs(o1.energy - o0.energy) / abs(o0.energy),
-               energy=o1.energy, norm=o1.norm)
+               energy=o1.energy, norm=o1.norm, traversal=tune)
     log("%s: %d steps in %.3fs, kernel %.4f ms/sweep, %.3e node-updates/s, %.1f GB/s" % (
         "fp64" if cfg.precision else "fp32", steps, wall, ms_sweep, res["node_updates_per_s"], res["achieved_gbs"]))
     return res, psi
@@ -313,9 +411,10 @@ def main():
     ap.add_argument("--steps", type=int, default=None,
                     help="timed steps (default: 20000 spin1, 10000 scalar, 5000 ensemble)")
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", choices=["spin1", "scalar", "ensemble"], default="spin1")
+    ap.add_argument("--workload", choices=["spin1", "scalar", "ensemble", "adjoint"], default="spin1")
     ap.add_argument("--graph-sweeps", type=int, default=20)
-    ap.add_argument("--tune-sweeps", type=int, default=6)
+    ap.add_argument("--tune-sweeps", type=int, default=10,
+                    help="autotuner trial sweeps per traversal candidate (0: default traversal)")
     ap.add_argument("--itp-steps", type=int, default=1000)
     ap.add_argument("--itp-dt", type=float, default=2e-4)
     ap.add_argument("--no-cpu", action="store_true")
@@ -328,13 +427,16 @@ def main():
         sys.exit(2)
     log = (lambda *a: None) if args.quiet else (lambda m: print("[bench] " + m, file=sys.stderr, flush=True))
     if args.steps is None:
-        args.steps = {"spin1": 20000, "scalar": 10000, "ensemble": 5000}[args.workload]
+        args.steps = {"spin1": 20000, "scalar": 10000, "ensemble": 5000, "adjoint": 4000}[args.workload]
     if args.workload == "ensemble" and (args.graph_sweeps <= 0 or args.graph_sweeps % 2 or
                                         args.steps % args.graph_sweeps):
         ap.error("--graph-sweeps must be even and divide --steps")
 
     if args.workload == "ensemble":
         print(json.dumps(run_ensemble(args, log)))
+        return
+    if args.workload == "adjoint":
+        print(json.dumps(run_adjoint(args, log)))
         return
 
     from spacetime_amd import gpe
@@ -348,12 +450,13 @@ def main():
         log("device copy bandwidth (torch copy_, 2 GiB): %.1f GB/s" % copy_gbs)
     psi0 = prepare_state(cfg, knots, args.itp_steps, args.itp_dt, log)
 
-    r32, psi32 = run_form(dataclasses.replace(cfg, precision=gpe.FP32), psi0, knots, args.warmup, args.steps, log)
+    r32, psi32 = run_form(dataclasses.replace(cfg, precision=gpe.FP32), psi0, knots, args.warmup, args.steps, log,
+                          args.tune_sweeps)
     acc = {}
     r64 = None
     if not args.no_fp64:
         r64, psi64 = run_form(dataclasses.replace(cfg, precision=gpe.FP64), psi0, knots, args.warmup, args.steps,
-                              log)
+                              log, args.tune_sweeps)
         acc["rel_l2_fp32_vs_fp64"] = float(np.linalg.norm((psi32 - psi64).ravel()) / np.linalg.norm(psi64.ravel()))
         del psi64
     acc.update(norm_drift_fp32=r32["norm_drift"], energy_drift_fp32=r32["energy_drift"])
@@ -387,6 +490,7 @@ def main():
             "frac": r32["achieved_gbs"] / PEAK_HBM_GBS,
             "traffic": PMC_TRAFFIC.get(info["workload"]),
             "kernel_ms_per_sweep": r32["kernel_ms_per_sweep"],
+            "traversal": r32["traversal"],
             "copy_gbs": copy_gbs,
             "frac_of_copy": (r32["achieved_gbs"] / copy_gbs) if copy_gbs else None,
             "algorithmic_bytes_per_sweep": r32["bytes_per_sweep"],

@@ -54,24 +54,31 @@ struct WaveCoord {
     bool valid;
 };
 
+__device__ __forceinline__ unsigned fast_div(unsigned n, const FastDiv &f)
+{
+    if (f.d == 1) return n;
+    const unsigned t = __umulhi(n, f.m);
+    return (t + ((n - t) >> 1)) >> (f.l - 1);
+}
+
 // Wave coordinates of workgroup b of nb (b, nb: the workgroup's place in its own run).
+// Traversal order, slowest to fastest: y-tile, z, row in tile, unknown v, x-segment.
 __device__ __forceinline__ WaveCoord decode(const Geometry &g, int b, int nb)
 {
     WaveCoord wc;
-    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const long long w = (long long)(g.xcd_remap ? logical_block(b, nb) : b) * 4 + wid;
-    long long r = w;
-    const int xseg = (int)(r % g.nseg);
-    r /= g.nseg;
-    wc.v = (int)(r % g.V);
-    r /= g.V;
-    const int yin = (int)(r % g.ty);
-    r /= g.ty;
-    wc.z = (int)(r % g.nz);
-    const int tile = (int)(r / g.nz);
-    wc.y = tile * g.ty + yin;
-    wc.valid = (w < g.waves) && (wc.y < g.ny);
-    wc.x = xseg * 64 + (int)(threadIdx.x & 63);
+    const unsigned wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const unsigned w = (unsigned)(g.xcd_remap ? logical_block(b, nb) : b) * 4u + wid;
+    const unsigned q0 = fast_div(w, g.f_nseg);
+    const int xseg = (int)(w - q0 * (unsigned)g.nseg);
+    const unsigned q1 = fast_div(q0, g.f_V);
+    wc.v = (int)(q0 - q1 * (unsigned)g.V);
+    const unsigned q2 = fast_div(q1, g.f_ty);
+    const int yin = (int)(q1 - q2 * (unsigned)g.ty);
+    const unsigned q3 = fast_div(q2, g.f_nz);
+    wc.z = (int)(q2 - q3 * (unsigned)g.nz);
+    wc.y = (int)q3 * g.ty + yin;
+    wc.valid = ((long long)w < g.waves) && (wc.y < g.ny);
+    wc.x = (xseg * 64 + (int)(threadIdx.x & 63)) * g.xpl;
     return wc;
 }
 

@@ -8,6 +8,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -64,7 +65,8 @@ void ramp_channel(const std::vector<double> &val, const std::vector<double> &dur
 struct gpe_ctx {
     gpe_config cfg{};
     const LatticeTable *lat = nullptr;
-    Geometry g{};
+    Geometry g{};    // one x per lane: observables, packing, adjoint
+    Geometry gs{};   // the sweep's geometry (1 or 2 x per lane)
     Physics ph{};
     int fp64 = 0;
     size_t elem_bytes = 0;   // bytes of one complex element
@@ -198,10 +200,40 @@ int upload_fields(gpe_ctx *c, long long first, int n)
     return GPE_OK;
 }
 
+FastDiv make_fast_div(unsigned d)
+{
+    FastDiv f;
+    f.d = d;
+    unsigned l = 0;
+    while ((1ull << l) < d) ++l;
+    f.l = l;
+    f.m = d <= 1 ? 1u : (unsigned)((((1ull << 32) * ((1ull << l) - d)) / d) + 1);
+    return f;
+}
+
+// Segments, tiles, waves and workgroups of a traversal from nx, ny, nz, V, ty and xpl.
+// Wave indices must fit in 32 bits (the decode's arithmetic).
+bool finish_geometry(Geometry &g)
+{
+    const int per_wave = 64 * g.xpl;
+    g.nseg = (g.nx + per_wave - 1) / per_wave;
+    g.ty = g.ty < g.ny ? g.ty : g.ny;
+    g.ntile = (g.ny + g.ty - 1) / g.ty;
+    g.waves = (long long)g.ntile * g.ty * g.nz * g.V * g.nseg;
+    const long long blocks = (g.waves + 3) / 4;
+    if (blocks >= (1ll << 30)) return false;
+    g.blocks = (int)blocks;
+    g.f_nseg = make_fast_div((unsigned)g.nseg);
+    g.f_V = make_fast_div((unsigned)g.V);
+    g.f_ty = make_fast_div((unsigned)g.ty);
+    g.f_nz = make_fast_div((unsigned)g.nz);
+    return true;
+}
+
 SweepArgs sweep_args(gpe_ctx *c, int mode_coef_sweep, const void *cur, void *next, double coef)
 {
     SweepArgs a;
-    a.g = c->g;
+    a.g = c->gs;
     a.tb = tables(c);
     a.ph = c->ph;
     a.cur = cur;
@@ -262,19 +294,17 @@ int gpe_create(const gpe_config *cfg, gpe_ctx **out)
     g.nz = cfg->nz;
     g.V = c->lat->V;
     g.deg = c->lat->deg;
-    g.nseg = (cfg->nx + 63) / 64;
+    g.xpl = 1;
     g.ty = cfg->ny < 8 ? cfg->ny : 8;
-    g.ntile = (cfg->ny + g.ty - 1) / g.ty;
     const long long align = c->fp64 ? 8 : 16;  // 128-byte line starts
     g.lx = ((long long)cfg->nx + align - 1) / align * align;
-    g.waves = (long long)g.ntile * g.ty * g.nz * g.V * g.nseg;
-    const long long blocks = (g.waves + 3) / 4;
-    if (blocks > 0x7fffffffLL) {
+    g.xcd_remap = 1;
+    c->gs = g;
+    c->gs.xpl = default_xpl(c->fp64, g.deg, cfg->ncomp, cfg->nx);
+    if (!finish_geometry(g) || !finish_geometry(c->gs)) {
         delete c;
         return GPE_EINVAL;
     }
-    g.blocks = (int)blocks;
-    g.xcd_remap = 1;
     c->level_elems = (long long)cfg->ncomp * g.nz * g.ny * g.V * g.lx;
 
     Physics &ph = c->ph;
@@ -573,7 +603,7 @@ int gpe_itp(gpe_ctx *c, int nsteps, double dt_itp)
         SweepArgs a = sweep_args(c, 0, c->level[0], c->level[1], dt_itp);
         a.scale_in = scale[i & 1];
         e = launch_sweep(c->fp64, c->cfg.ncomp, MODE_ITP, a, c->stream);
-        if (e == hipSuccess) e = launch_finalize(c->d_partials, c->g.blocks, 1, -1, norm, c->stream);
+        if (e == hipSuccess) e = launch_finalize(c->d_partials, c->gs.blocks, 1, -1, norm, c->stream);
         if (e == hipSuccess) e = launch_norm_to_scale(norm, 1.0, scale[(i + 1) & 1], c->stream);
         if (e != hipSuccess) return hip_fail(c, e, "ITP step");
         std::swap(c->level[0], c->level[1]);
@@ -979,12 +1009,21 @@ int gpe_adjoint(gpe_ctx *c, const gpe_adjoint_spec *spec, gpe_adjoint_result *re
 
 // ------------------------------------------------------------------------- ensembles ---
 
+// Members sharing (face degree, x per lane) form a group launched by one specialised kernel.
+struct EnsembleGroup {
+    int deg, xpl;
+    int first, count;   // members [first, first + count) of the sorted member list
+    int blocks;         // workgroups of the group's launch
+};
+
 struct gpe_ensemble {
-    std::vector<gpe_ctx *> m;
+    std::vector<gpe_ctx *> m;          // sorted by group
+    std::vector<Geometry> trav;        // each member's sweep traversal when the ensemble was made
     std::vector<EnsembleMember> h;
+    std::vector<EnsembleGroup> groups;
     EnsembleMember *d = nullptr;
     int *d_base = nullptr;
-    int blocks = 0, ncomp = 0, fp64 = 0, per = 1;
+    int ncomp = 0, fp64 = 0, per = 1;
     hipStream_t stream = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -1016,18 +1055,28 @@ int gpe_ensemble_create(gpe_ctx *const *members, int n, gpe_ensemble **out)
     gpe_ensemble *e = new (std::nothrow) gpe_ensemble();
     if (!e) return GPE_ENOMEM;
     e->m.assign(members, members + n);
+    std::stable_sort(e->m.begin(), e->m.end(), [](const gpe_ctx *a, const gpe_ctx *b) {
+        return a->gs.deg != b->gs.deg ? a->gs.deg < b->gs.deg : a->gs.xpl < b->gs.xpl;
+    });
     e->ncomp = c0->cfg.ncomp;
     e->fp64 = c0->fp64;
     e->per = c0->cfg.scheme == GPE_SCHEME_STAGGERED ? 2 : 1;
     e->h.resize(n);
     int begin = 0;
     for (int i = 0; i < n; ++i) {
+        const gpe_ctx *c = e->m[i];
+        if (i == 0 || c->gs.deg != e->groups.back().deg || c->gs.xpl != e->groups.back().xpl) {
+            e->groups.push_back(EnsembleGroup{c->gs.deg, c->gs.xpl, i, 0, 0});
+            begin = 0;
+        }
+        e->groups.back().count += 1;
         e->h[i].begin = begin;
-        e->h[i].blocks = members[i]->g.blocks;
-        begin += (members[i]->g.blocks + 7) / 8 * 8;
-        e->unknowns += (double)members[i]->cfg.nx * members[i]->cfg.ny * members[i]->cfg.nz * members[i]->g.V;
+        e->h[i].blocks = c->gs.blocks;
+        begin += (c->gs.blocks + 7) / 8 * 8;
+        e->groups.back().blocks = begin;
+        e->trav.push_back(c->gs);
+        e->unknowns += (double)c->cfg.nx * c->cfg.ny * c->cfg.nz * c->g.V;
     }
-    e->blocks = begin;
     int rc = GPE_OK;
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) rc = GPE_EDEVICE;
     if (!rc && hipMalloc((void **)&e->d, sizeof(EnsembleMember) * n) != hipSuccess) rc = GPE_ENOMEM;
@@ -1075,6 +1124,9 @@ int gpe_ensemble_step(gpe_ensemble *e, int nsteps, int graph_sweeps)
     for (size_t i = 0; i < e->m.size(); ++i) {
         gpe_ctx *c = e->m[i];
         if (!c->level1_valid) return ens_fail(e, GPE_ESTATE, "member not started");
+        const Geometry &t = e->trav[i];
+        if (c->gs.blocks != t.blocks || c->gs.xpl != t.xpl || c->gs.ty != t.ty || c->gs.xcd_remap != t.xcd_remap)
+            return ens_fail(e, GPE_ESTATE, "member traversal changed after the ensemble was created");
         int rc = upload_fields(c, c->sweeps, nsw);
         if (rc) return ens_fail(e, rc, c->err);
         ENSCALL(e, hipStreamSynchronize(c->stream));
@@ -1084,9 +1136,20 @@ int gpe_ensemble_step(gpe_ensemble *e, int nsteps, int graph_sweeps)
     }
     ENSCALL(e, hipMemcpyAsync(e->d, e->h.data(), sizeof(EnsembleMember) * e->h.size(), hipMemcpyHostToDevice,
                               e->stream));
-    EnsembleArgs a;
-    a.members = e->d;
-    a.nmember = (int)e->h.size();
+    // one launch per group per sweep; the groups of a sweep are independent
+    auto launch_all = [&](int k, const int *base) -> hipError_t {
+        for (const EnsembleGroup &gr : e->groups) {
+            EnsembleArgs a;
+            a.members = e->d + gr.first;
+            a.nmember = gr.count;
+            a.parity = k & 1;
+            a.sweep = k;
+            a.sweep_base = base;
+            hipError_t err = launch_ensemble_sweep(e->fp64, e->ncomp, gr.deg, gr.xpl, a, gr.blocks, e->stream);
+            if (err != hipSuccess) return err;
+        }
+        return hipSuccess;
+    };
     ENSCALL(e, hipEventRecord(e->ev0, e->stream));
     if (graph_sweeps > 0) {
         if (!e->gexec || e->graph_sweeps != graph_sweeps) {
@@ -1095,26 +1158,24 @@ int gpe_ensemble_step(gpe_ensemble *e, int nsteps, int graph_sweeps)
             e->gexec = nullptr;
             e->graph = nullptr;
             ENSCALL(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-            for (int k = 0; k < graph_sweeps; ++k) {
-                a.parity = k & 1;
-                a.sweep = k;
-                a.sweep_base = e->d_base;
-                (void)launch_ensemble_sweep(e->fp64, e->ncomp, a, e->blocks, e->stream);
+            hipError_t cap = hipSuccess;
+            for (int k = 0; k < graph_sweeps && cap == hipSuccess; ++k) cap = launch_all(k, e->d_base);
+            if (cap == hipSuccess) cap = launch_advance_base(e->d_base, graph_sweeps, e->stream);
+            hipGraph_t captured = nullptr;
+            const hipError_t end = hipStreamEndCapture(e->stream, &captured);
+            if (cap != hipSuccess || end != hipSuccess) {
+                if (captured) (void)hipGraphDestroy(captured);
+                return ens_fail(e, GPE_EDEVICE, std::string("graph capture: ") +
+                                                    hipGetErrorString(cap != hipSuccess ? cap : end));
             }
-            (void)launch_advance_base(e->d_base, graph_sweeps, e->stream);
-            ENSCALL(e, hipStreamEndCapture(e->stream, &e->graph));
+            e->graph = captured;
             ENSCALL(e, hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
             e->graph_sweeps = graph_sweeps;
         }
         ENSCALL(e, launch_set_base(e->d_base, 0, e->stream));
         for (int r = 0; r < nsw / graph_sweeps; ++r) ENSCALL(e, hipGraphLaunch(e->gexec, e->stream));
     } else {
-        for (int k = 0; k < nsw; ++k) {
-            a.parity = k & 1;
-            a.sweep = k;
-            a.sweep_base = nullptr;
-            ENSCALL(e, launch_ensemble_sweep(e->fp64, e->ncomp, a, e->blocks, e->stream));
-        }
+        for (int k = 0; k < nsw; ++k) ENSCALL(e, launch_all(k, nullptr));
     }
     ENSCALL(e, hipEventRecord(e->ev1, e->stream));
     ENSCALL(e, hipEventSynchronize(e->ev1));
@@ -1138,34 +1199,66 @@ int gpe_ensemble_timing(const gpe_ensemble *e, double *ms_per_sweep, double *unk
 
 // ------------------------------------------------------------------------ autotuning ---
 
-static void set_traversal(gpe_ctx *c, int ty, int remap)
+static void set_traversal(gpe_ctx *c, int ty, int remap, int xpl)
 {
-    Geometry &g = c->g;
-    g.ty = ty < g.ny ? ty : g.ny;
-    g.ntile = (g.ny + g.ty - 1) / g.ty;
-    g.waves = (long long)g.ntile * g.ty * g.nz * g.V * g.nseg;
-    g.blocks = (int)((g.waves + 3) / 4);
-    g.xcd_remap = remap;
+    c->gs.xpl = xpl;
+    for (Geometry *g : {&c->g, &c->gs}) {
+        g->ty = ty;
+        g->xcd_remap = remap;
+        (void)finish_geometry(*g);  // cannot overflow: at most ny + 31 rows per z-plane
+    }
+}
+
+// Reduction partials are sized by the workgroups of the one-x-per-lane traversal.
+static int grow_partials(gpe_ctx *c, int blocks_before)
+{
+    if (c->g.blocks <= blocks_before) return GPE_OK;
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(c->d_partials);
+    c->d_partials = nullptr;
+    if (hipMalloc((void **)&c->d_partials, sizeof(double) * OBS_COUNT * (size_t)c->g.blocks) != hipSuccess)
+        return fail(c, GPE_ENOMEM, "partials");
+    return GPE_OK;
+}
+
+int gpe_set_traversal(gpe_ctx *c, int tile_rows, int xcd_remap, int x_per_lane)
+{
+    if (!c) return GPE_EINVAL;
+    if (tile_rows < 1 || tile_rows > 64 || (xcd_remap != 0 && xcd_remap != 1) || (x_per_lane != 1 && x_per_lane != 2))
+        return fail(c, GPE_EINVAL, "bad traversal");
+    if (x_per_lane == 2 && !pair_lanes_ok(c->fp64, c->g.deg, c->cfg.ncomp))
+        return fail(c, GPE_EINVAL, "two x per lane needs FP32 and degree x components <= 20");
+    const int before = c->g.blocks;
+    set_traversal(c, tile_rows, xcd_remap, x_per_lane);
+    return grow_partials(c, before);
+}
+
+int gpe_traversal(const gpe_ctx *c, int *tile_rows, int *xcd_remap, int *x_per_lane)
+{
+    if (!c) return GPE_EINVAL;
+    if (tile_rows) *tile_rows = c->gs.ty;
+    if (xcd_remap) *xcd_remap = c->gs.xcd_remap;
+    if (x_per_lane) *x_per_lane = c->gs.xpl;
+    return GPE_OK;
 }
 
 typedef std::tuple<int, int, int, int, int, int> TuneKey;
-static std::map<TuneKey, std::pair<int, int>> g_tune_cache;
+static std::map<TuneKey, std::tuple<int, int, int>> g_tune_cache;  // rows, remap, x per lane
 static std::mutex g_tune_mutex;
 
-int gpe_autotune(gpe_ctx *c, int trial_sweeps, int *tile_rows, int *xcd_remap, double *ms_out)
+int gpe_autotune(gpe_ctx *c, int trial_sweeps, int *tile_rows, int *xcd_remap, int *x_per_lane, double *ms_out)
 {
     if (!c) return GPE_EINVAL;
     if (trial_sweeps < 0) return fail(c, GPE_EINVAL, "negative trial sweeps");
     const TuneKey key(c->cfg.lattice, c->cfg.ncomp, c->fp64, c->cfg.nx, c->cfg.ny, c->cfg.nz);
     const int max_blocks_before = c->g.blocks;
-    int best_ty = c->g.ty, best_remap = c->g.xcd_remap;
+    int best_ty = c->g.ty, best_remap = c->g.xcd_remap, best_xpl = c->gs.xpl;
     double best_ms = -1.0;
     {
         std::lock_guard<std::mutex> lk(g_tune_mutex);
         auto it = g_tune_cache.find(key);
         if (it != g_tune_cache.end()) {
-            best_ty = it->second.first;
-            best_remap = it->second.second;
+            std::tie(best_ty, best_remap, best_xpl) = it->second;
             trial_sweeps = 0;
         }
     }
@@ -1182,10 +1275,12 @@ int gpe_autotune(gpe_ctx *c, int trial_sweeps, int *tile_rows, int *xcd_remap, d
         int rc = upload_fields(c, c->sweeps, 1);
         if (rc) return rc;
         const int tys[] = {1, 2, 4, 8, 16, 32};
+        const int nxpl = pair_lanes_ok(c->fp64, c->g.deg, c->cfg.ncomp) ? 2 : 1;
+        for (int xpl = 1; xpl <= nxpl; ++xpl)
         for (int ty : tys) {
             if (ty > c->g.ny && ty != 1) continue;
             for (int remap = 0; remap < 2; ++remap) {
-                set_traversal(c, ty, remap);
+                set_traversal(c, ty, remap, xpl);
                 void *cur = s0, *nxt = s1;
                 for (int k = 0; k < trial_sweeps + 2; ++k) {
                     if (k == 2) HIPCALL(c, hipEventRecord(c->ev0, c->stream));
@@ -1203,6 +1298,7 @@ int gpe_autotune(gpe_ctx *c, int trial_sweeps, int *tile_rows, int *xcd_remap, d
                     best_ms = per;
                     best_ty = c->g.ty;
                     best_remap = remap;
+                    best_xpl = xpl;
                 }
             }
         }
@@ -1210,18 +1306,14 @@ int gpe_autotune(gpe_ctx *c, int trial_sweeps, int *tile_rows, int *xcd_remap, d
         (void)hipFree(s1);
         (void)parts;
         std::lock_guard<std::mutex> lk(g_tune_mutex);
-        g_tune_cache[key] = std::make_pair(best_ty, best_remap);
+        g_tune_cache[key] = std::make_tuple(best_ty, best_remap, best_xpl);
     }
-    set_traversal(c, best_ty, best_remap);
-    if (c->g.blocks > max_blocks_before) {  // partials are sized by workgroups
-        HIPCALL(c, hipStreamSynchronize(c->stream));
-        (void)hipFree(c->d_partials);
-        c->d_partials = nullptr;
-        if (hipMalloc((void **)&c->d_partials, sizeof(double) * OBS_COUNT * (size_t)c->g.blocks) != hipSuccess)
-            return fail(c, GPE_ENOMEM, "partials");
-    }
+    set_traversal(c, best_ty, best_remap, best_xpl);
+    int rc = grow_partials(c, max_blocks_before);
+    if (rc) return rc;
     if (tile_rows) *tile_rows = best_ty;
     if (xcd_remap) *xcd_remap = best_remap;
+    if (x_per_lane) *x_per_lane = best_xpl;
     if (ms_out) *ms_out = best_ms;
     return GPE_OK;
 }

@@ -15,9 +15,19 @@ namespace gpe {
 // the neighbour rows are uniform across the wave and every neighbour load is a contiguous
 // 64-lane access.  Out-of-box neighbours read as zero (Dirichlet, as the reference's
 // never-written halo).
+// Division of a 32-bit unsigned n by an invariant d >= 1 as a multiply-high and shifts
+// (Granlund & Montgomery 1994; Warren, Hacker's Delight, section 10-8): with l = ceil(log2 d)
+// and m = floor(2^32 (2^l - d) / d) + 1, n / d = (t + ((n - t) >> 1)) >> (l - 1), t = mulhi(n, m),
+// exact for every n < 2^32.  d = 1 is passed through.
+struct FastDiv {
+    unsigned d, m, l;
+};
+
 struct Geometry {
     int nx, ny, nz, V, deg;
-    int nseg;          // ceil(nx / 64) wave segments per x-line
+    FastDiv f_nseg, f_V, f_ty, f_nz;  // the decode's divisors
+    int xpl;           // consecutive x values per lane (1, or 2 for the FP32 sweep)
+    int nseg;          // ceil(nx / (64 xpl)) wave segments per x-line
     int ty;            // y rows per tile of the traversal order
     int ntile;         // ceil(ny / ty)
     long long lx;      // elements per x-line
@@ -25,6 +35,23 @@ struct Geometry {
     int blocks;        // workgroups of a sweep (4 waves each)
     int xcd_remap;     // give each XCD a contiguous range of the traversal order
 };
+
+// x values per lane of the sweep kernel.  Two (one 16-byte load per component and neighbour,
+// the access width of the FP64 form) exist for FP32 where the DEG * C neighbour pairs fit the
+// register budget; which one runs is a traversal choice (default_xpl, then the autotuner).
+__host__ __device__ constexpr bool pair_lanes_ok(int fp64, int deg, int ncomp)
+{
+    return !fp64 && deg * ncomp <= 20;
+}
+
+// Default: pairs when allowed and they leave no more idle lanes than single values do
+// (x-lines of 48 or 64 would idle half of each wave), or for one component, where the
+// per-wave overhead of single values dominates.
+__host__ __device__ constexpr int default_xpl(int fp64, int deg, int ncomp, int nx)
+{
+    return (pair_lanes_ok(fp64, deg, ncomp) &&
+            (ncomp == 1 || (nx + 127) / 128 * 128 <= (nx + 63) / 64 * 64)) ? 2 : 1;
+}
 
 // Per-context constant tables on the device (weights already scaled by -1/2 / s^2).
 struct Tables {
@@ -142,7 +169,9 @@ struct EnsembleArgs {
     const int *sweep_base;    // device counter added to sweep (graph replays), may be null
 };
 
-hipError_t launch_ensemble_sweep(int fp64, int ncomp, const EnsembleArgs &e, int blocks, hipStream_t s);
+// All members of one launch share the component count, precision, face degree and lanes.
+hipError_t launch_ensemble_sweep(int fp64, int ncomp, int deg, int xpl, const EnsembleArgs &e, int blocks,
+                                 hipStream_t s);
 hipError_t launch_advance_base(int *base, int by, hipStream_t s);
 hipError_t launch_set_base(int *base, int value, hipStream_t s);
 

@@ -19,8 +19,9 @@ namespace gpe {
 
 // ------------------------------------------------------------------------- the sweep --
 
+// One x value per lane (FP64, and FP32 where the neighbour pairs would not fit).
 template <typename R, int C, int DEG, int MODE>
-__device__ __forceinline__ void sweep_body(const SweepArgs &a, int sweep, int b, int nb)
+__device__ __forceinline__ void sweep_body_single(const SweepArgs &a, int sweep, int b, int nb)
 {
     using V2 = vec2_t<R>;
     const Geometry &g = a.g;
@@ -161,16 +162,215 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, int sweep, int b,
     }
 }
 
-template <typename R, int C, int DEG, int MODE>
-__global__ __launch_bounds__(256) void k_sweep(SweepArgs a)
+// Two consecutive x values per lane (FP32): every access is one 16-byte load or store of a
+// pair of complex values.  Lane l of segment s owns x0 = 128 s + 2 l and x0 + 1.  A
+// neighbour pair starts at x0 + dx, which is 8-byte aligned only; the start is clamped to
+// [0, lx - 2] (inside the line, so inside the allocation) and the two values are shifted
+// back into place, and any value outside the box (x, y or z) is replaced by zero by a
+// select, so the line padding x >= nx is never used whatever it holds.
+typedef float f4a __attribute__((ext_vector_type(4)));
+typedef float f4u __attribute__((ext_vector_type(4), aligned(8)));
+typedef float f2n __attribute__((ext_vector_type(2)));
+
+template <int C, int DEG, int MODE>
+__device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, int b, int nb)
 {
-    sweep_body<R, C, DEG, MODE>(a, a.sweep, blockIdx.x, gridDim.x);
+    const Geometry &g = a.g;
+    const WaveCoord wc = decode(g, b, nb);
+    double part = 0.0;
+
+    if (wc.valid) {
+        const int x0 = wc.x, v = wc.v, y = wc.y, z = wc.z;
+        const bool act0 = x0 < g.nx, act1 = x0 + 1 < g.nx;
+        const int xc = act0 ? x0 : ((g.nx - 1) & ~1);  // even, so xc + 1 <= lx - 1
+        const float2 *__restrict__ cur = (const float2 *)a.cur;
+        float2 *__restrict__ nxt = (float2 *)a.next;
+        const float2 *__restrict__ prv = (const float2 *)(a.prev ? a.prev : a.next);
+        const long long cstride = (long long)g.nz * g.ny * g.V * g.lx;
+        const long long own_line = line_index(g, 0, z, y, v);
+        const int lxm2 = (int)g.lx - 2;
+
+        float pr[2][C], pi[2][C], qr[2][C], qi[2][C], ar[2][C], ai[2][C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const f4a o = *(const f4a *)(cur + own_line + c * cstride + xc);
+            pr[0][c] = o.x;
+            pi[0][c] = o.y;
+            pr[1][c] = o.z;
+            pi[1][c] = o.w;
+            ar[0][c] = ar[1][c] = 0.f;
+            ai[0][c] = ai[1][c] = 0.f;
+        }
+        if constexpr (MODE == MODE_UPDATE) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const f4a o = __builtin_nontemporal_load((const f4a *)(prv + own_line + c * cstride + xc));
+                qr[0][c] = o.x;
+                qi[0][c] = o.y;
+                qr[1][c] = o.z;
+                qi[1][c] = o.w;
+            }
+        }
+        const int *face = a.tb.face + v * DEG;
+        const float *wt = a.tb.w32 + v * DEG;
+        const float2 *nbase[DEG];
+        int sh[DEG];
+        bool ok0[DEG], ok1[DEG];
+#pragma unroll
+        for (int f = 0; f < DEG; ++f) {
+            const int pk = face[f];
+            const int dx = (pk & 3) - 1, dy = ((pk >> 2) & 3) - 1, dz = ((pk >> 4) & 3) - 1, t = pk >> 8;
+            int zn = z + dz, yn = y + dy;
+            const bool inside = (unsigned)zn < (unsigned)g.nz && (unsigned)yn < (unsigned)g.ny;
+            if (!inside) {
+                zn = z;
+                yn = y;
+            }
+            const int s0 = xc + dx;
+            const int sc = s0 < 0 ? 0 : (s0 > lxm2 ? lxm2 : s0);
+            sh[f] = s0 - sc;  // -1, 0 or +1
+            ok0[f] = inside && (unsigned)(x0 + dx) < (unsigned)g.nx;
+            ok1[f] = inside && (unsigned)(x0 + 1 + dx) < (unsigned)g.nx;
+            nbase[f] = cur + line_index(g, 0, zn, yn, t) + sc;
+        }
+        f4u nbv[DEG][C];
+#pragma unroll
+        for (int f = 0; f < DEG; ++f)
+#pragma unroll
+            for (int c = 0; c < C; ++c) nbv[f][c] = *(const f4u *)(nbase[f] + c * cstride);
+#pragma unroll
+        for (int f = 0; f < DEG; ++f) {
+            const float w = wt[f];
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const f4u e = nbv[f][c];
+                // values at x0 + dx and x0 + 1 + dx from the clamped pair
+                float n0r = sh[f] == 0 ? e.x : e.z, n0i = sh[f] == 0 ? e.y : e.w;
+                float n1r = sh[f] == 0 ? e.z : e.x, n1i = sh[f] == 0 ? e.w : e.y;
+                n0r = (ok0[f] && sh[f] >= 0) ? n0r : 0.f;
+                n0i = (ok0[f] && sh[f] >= 0) ? n0i : 0.f;
+                n1r = (ok1[f] && sh[f] <= 0) ? n1r : 0.f;
+                n1i = (ok1[f] && sh[f] <= 0) ? n1i : 0.f;
+                ar[0][c] += w * n0r;
+                ai[0][c] += w * n0i;
+                ar[1][c] += w * n1r;
+                ai[1][c] += w * n1i;
+            }
+        }
+        const float wsum = a.tb.wsum32[v];
+        float sc = 1.f;
+        if constexpr (MODE == MODE_ITP) sc = (float)(*a.scale_in);
+        const Physics &ph = a.ph;
+        const FieldRec &fr = a.fields[sweep];
+        const float *pos = a.tb.pos32 + 3 * v;
+        const float cell = (float)ph.cell;
+        const float ry = (float)ph.p0[1] + cell * (float)y + pos[1];
+        const float rz = (float)ph.p0[2] + cell * (float)z + pos[2];
+        const bool trap_on = fr.t < ph.trap_off_time;
+        const float coef = (float)a.coef;
+        float2 out[2][C];
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            float hr[C], hi[C];
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                hr[c] = sc * (ar[e][c] - wsum * pr[e][c]);
+                hi[c] = sc * (ai[e][c] - wsum * pi[e][c]);
+                pr[e][c] *= sc;
+                pi[e][c] *= sc;
+            }
+            const float rx = (float)ph.p0[0] + cell * (float)(x0 + e) + pos[0];
+            float vtrap = 0.f;
+            if (trap_on)
+                vtrap = 0.5f * ((float)ph.trap2[0] * rx * rx + (float)ph.trap2[1] * ry * ry +
+                                (float)ph.trap2[2] * rz * rz) + (float)ph.trap_offset;
+            float Bx = 0, By = 0, Bz = 0, Qx = 0, Qy = 0, Qz = 0;
+            if constexpr (C > 1) {
+                const float dxr = rx - (float)ph.center[0], dyr = ry - (float)ph.center[1],
+                            dzr = rz - (float)ph.center[2];
+                const float bq = (float)fr.bq;
+                Bx = bq * dxr + (float)fr.bb[0];
+                By = bq * dyr + (float)fr.bb[1];
+                Bz = -2.f * bq * dzr + (float)fr.bb[2];
+                const float bqq = (float)fr.bqq;
+                Qx = bqq * dxr + (float)fr.bbq[0];
+                Qy = bqq * dyr + (float)fr.bbq[1];
+                Qz = -2.f * bqq * dzr + (float)fr.bbq[2];
+            }
+            add_local_h<float, C>(pr[e], pi[e], vtrap, Bx, By, Bz, Qx, Qy, Qz, (float)ph.c0, (float)ph.c1,
+                                  (float)ph.c2, (float)ph.alpha, (float)ph.quad_sign, (float)ph.q, a.quad, a.loss,
+                                  hr, hi);
+            const bool live = e == 0 ? act0 : act1;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                float2 o;
+                if constexpr (MODE == MODE_UPDATE) {
+                    o.x = qr[e][c] + coef * hi[c];
+                    o.y = qi[e][c] - coef * hr[c];
+                } else if constexpr (MODE == MODE_EULER) {
+                    o.x = pr[e][c] + coef * hi[c];
+                    o.y = pi[e][c] - coef * hr[c];
+                } else if constexpr (MODE == MODE_ITP) {
+                    o.x = pr[e][c] - coef * hr[c];
+                    o.y = pi[e][c] - coef * hi[c];
+                    if (live) part += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
+                } else {
+                    o.x = hr[c];
+                    o.y = hi[c];
+                }
+                out[e][c] = o;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            float2 *dst = nxt + own_line + c * cstride + xc;
+            if (act1) {
+                f4a o;
+                o.x = out[0][c].x;
+                o.y = out[0][c].y;
+                o.z = out[1][c].x;
+                o.w = out[1][c].y;
+                __builtin_nontemporal_store(o, (f4a *)dst);
+            } else if (act0) {
+                f2n o;
+                o.x = out[0][c].x;
+                o.y = out[0][c].y;
+                __builtin_nontemporal_store(o, (f2n *)dst);
+            }
+        }
+        if constexpr (MODE == MODE_ITP) part *= a.tb.dv[v];
+    }
+
+    if constexpr (MODE == MODE_ITP) {
+        __shared__ double red[4];
+        part = wave_sum(part);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
+        __syncthreads();
+        if (threadIdx.x == 0) a.partials[b] = (red[0] + red[1]) + (red[2] + red[3]);
+    }
 }
 
-// One leapfrog sweep of every member of an ensemble in a single launch.  Member m owns
-// workgroups [begin_m, begin_m + blocks_m) (begin_m a multiple of 8, so the XCD remap of the
-// member's own range stays valid); its arguments for parity p are m.args[p].
-template <typename R, int C>
+template <typename R, int C, int DEG, int MODE, int X>
+__device__ __forceinline__ void sweep_body(const SweepArgs &a, int sweep, int b, int nb)
+{
+    if constexpr (X == 2)
+        sweep_body_pair<C, DEG, MODE>(a, sweep, b, nb);
+    else
+        sweep_body_single<R, C, DEG, MODE>(a, sweep, b, nb);
+}
+
+template <typename R, int C, int DEG, int MODE, int X>
+__global__ __launch_bounds__(256) void k_sweep(SweepArgs a)
+{
+    sweep_body<R, C, DEG, MODE, X>(a, a.sweep, blockIdx.x, gridDim.x);
+}
+
+// One leapfrog sweep of every member of an ensemble group in a single launch (a group: the
+// members sharing component count, face degree and lanes per x, so the kernel is as
+// specialised as a single run's).  Member m owns workgroups [begin_m, begin_m + blocks_m)
+// (begin_m a multiple of 8, so the XCD remap of the member's own range stays valid); its
+// arguments for parity p are m.args[p].
+template <typename R, int C, int DEG, int X>
 __global__ __launch_bounds__(256) void k_ensemble_sweep(EnsembleArgs e)
 {
     const int b = blockIdx.x;
@@ -187,11 +387,7 @@ __global__ __launch_bounds__(256) void k_ensemble_sweep(EnsembleArgs e)
     if (local >= m.blocks) return;  // padding workgroup (uniform)
     const SweepArgs &a = m.args[e.parity];
     const int sweep = e.sweep + (e.sweep_base ? *e.sweep_base : 0);
-    switch (a.g.deg) {
-    case 4: sweep_body<R, C, 4, MODE_UPDATE>(a, sweep, local, m.blocks); break;
-    case 6: sweep_body<R, C, 6, MODE_UPDATE>(a, sweep, local, m.blocks); break;
-    default: sweep_body<R, C, 8, MODE_UPDATE>(a, sweep, local, m.blocks); break;
-    }
+    sweep_body<R, C, DEG, MODE_UPDATE, X>(a, sweep, local, m.blocks);
 }
 
 __global__ void k_advance_base(int *base, int by)
@@ -597,18 +793,28 @@ __global__ __launch_bounds__(256) void k_pack(Geometry g, int ncomp, const doubl
 
 static inline unsigned grid1(long long n) { return (unsigned)((n + 255) / 256); }
 
-template <typename R, int C, int DEG>
-static hipError_t sweep_mode(int mode, const SweepArgs &a, hipStream_t s)
+template <typename R, int C, int DEG, int X>
+static hipError_t sweep_mode_x(int mode, const SweepArgs &a, hipStream_t s)
 {
     dim3 grid(a.g.blocks), block(256);
     switch (mode) {
-    case MODE_UPDATE: hipLaunchKernelGGL((k_sweep<R, C, DEG, MODE_UPDATE>), grid, block, 0, s, a); break;
-    case MODE_EULER: hipLaunchKernelGGL((k_sweep<R, C, DEG, MODE_EULER>), grid, block, 0, s, a); break;
-    case MODE_ITP: hipLaunchKernelGGL((k_sweep<R, C, DEG, MODE_ITP>), grid, block, 0, s, a); break;
-    case MODE_HPSI: hipLaunchKernelGGL((k_sweep<R, C, DEG, MODE_HPSI>), grid, block, 0, s, a); break;
+    case MODE_UPDATE: hipLaunchKernelGGL((k_sweep<R, C, DEG, MODE_UPDATE, X>), grid, block, 0, s, a); break;
+    case MODE_EULER: hipLaunchKernelGGL((k_sweep<R, C, DEG, MODE_EULER, X>), grid, block, 0, s, a); break;
+    case MODE_ITP: hipLaunchKernelGGL((k_sweep<R, C, DEG, MODE_ITP, X>), grid, block, 0, s, a); break;
+    case MODE_HPSI: hipLaunchKernelGGL((k_sweep<R, C, DEG, MODE_HPSI, X>), grid, block, 0, s, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <typename R, int C, int DEG>
+static hipError_t sweep_mode(int mode, const SweepArgs &a, hipStream_t s)
+{
+    if (a.g.xpl == 1) return sweep_mode_x<R, C, DEG, 1>(mode, a, s);
+    if constexpr (pair_lanes_ok(std::is_same<R, double>::value, DEG, C)) {
+        if (a.g.xpl == 2) return sweep_mode_x<R, C, DEG, 2>(mode, a, s);
+    }
+    return hipErrorInvalidValue;
 }
 
 template <typename R, int C>
@@ -667,22 +873,49 @@ hipError_t launch_observables(int fp64, int ncomp, const ObsArgs &a, hipStream_t
     return fp64 ? obs_comp<double>(ncomp, a, s) : obs_comp<float>(ncomp, a, s);
 }
 
-template <typename R>
-static hipError_t ens_comp(int ncomp, const EnsembleArgs &e, int blocks, hipStream_t s)
+template <typename R, int C, int DEG>
+static hipError_t ens_x(int xpl, const EnsembleArgs &e, int blocks, hipStream_t s)
 {
     dim3 grid(blocks), block(256);
-    switch (ncomp) {
-    case 1: hipLaunchKernelGGL((k_ensemble_sweep<R, 1>), grid, block, 0, s, e); break;
-    case 3: hipLaunchKernelGGL((k_ensemble_sweep<R, 3>), grid, block, 0, s, e); break;
-    case 5: hipLaunchKernelGGL((k_ensemble_sweep<R, 5>), grid, block, 0, s, e); break;
-    default: return hipErrorInvalidValue;
+    if (xpl == 1) {
+        hipLaunchKernelGGL((k_ensemble_sweep<R, C, DEG, 1>), grid, block, 0, s, e);
+        return hipGetLastError();
     }
-    return hipGetLastError();
+    if constexpr (pair_lanes_ok(std::is_same<R, double>::value, DEG, C)) {
+        if (xpl == 2) {
+            hipLaunchKernelGGL((k_ensemble_sweep<R, C, DEG, 2>), grid, block, 0, s, e);
+            return hipGetLastError();
+        }
+    }
+    return hipErrorInvalidValue;
 }
 
-hipError_t launch_ensemble_sweep(int fp64, int ncomp, const EnsembleArgs &e, int blocks, hipStream_t s)
+template <typename R, int C>
+static hipError_t ens_deg(int deg, int xpl, const EnsembleArgs &e, int blocks, hipStream_t s)
 {
-    return fp64 ? ens_comp<double>(ncomp, e, blocks, s) : ens_comp<float>(ncomp, e, blocks, s);
+    switch (deg) {
+    case 4: return ens_x<R, C, 4>(xpl, e, blocks, s);
+    case 6: return ens_x<R, C, 6>(xpl, e, blocks, s);
+    case 8: return ens_x<R, C, 8>(xpl, e, blocks, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename R>
+static hipError_t ens_comp(int ncomp, int deg, int xpl, const EnsembleArgs &e, int blocks, hipStream_t s)
+{
+    switch (ncomp) {
+    case 1: return ens_deg<R, 1>(deg, xpl, e, blocks, s);
+    case 3: return ens_deg<R, 3>(deg, xpl, e, blocks, s);
+    case 5: return ens_deg<R, 5>(deg, xpl, e, blocks, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_ensemble_sweep(int fp64, int ncomp, int deg, int xpl, const EnsembleArgs &e, int blocks,
+                                 hipStream_t s)
+{
+    return fp64 ? ens_comp<double>(ncomp, deg, xpl, e, blocks, s) : ens_comp<float>(ncomp, deg, xpl, e, blocks, s);
 }
 
 hipError_t launch_advance_base(int *base, int by, hipStream_t s)

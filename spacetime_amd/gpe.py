@@ -116,8 +116,12 @@ def load_library(path=LIB_PATH):
     lib.gpe_ensemble_last_error.restype = ctypes.c_char_p
     lib.gpe_ensemble_step.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     lib.gpe_ensemble_timing.argtypes = [vp, _dp, _dp]
-    lib.gpe_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _dp]
+    lib.gpe_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                 ctypes.POINTER(ctypes.c_int), _dp]
     lib.gpe_set_trap.argtypes = [vp, _dp]
+    lib.gpe_set_traversal.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.gpe_traversal.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                  ctypes.POINTER(ctypes.c_int)]
     lib.gpe_field_width.argtypes = [ctypes.c_int]
     lib.gpe_local_fields.argtypes = [vp, ctypes.c_int, ctypes.c_int, _dp]
     lib.gpe_fill_thomas_fermi.argtypes = [vp, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_double,
@@ -359,12 +363,23 @@ class Stepper:
         return t.value, n.value
 
     def autotune(self, trial_sweeps=20):
-        """Pick the fastest traversal (y-tile rows, XCD remap) for this context (gpe_autotune)."""
-        ty, rm = ctypes.c_int(), ctypes.c_int()
+        """Pick the fastest traversal (y-tile rows, XCD remap, x values per lane) for this context
+        (gpe_autotune)."""
+        ty, rm, xp = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         ms = ctypes.c_double()
         self._check(self.lib.gpe_autotune(self._h, int(trial_sweeps), ctypes.byref(ty), ctypes.byref(rm),
-                                          ctypes.byref(ms)), "gpe_autotune")
-        return dict(tile_rows=ty.value, xcd_remap=rm.value, ms_per_sweep=ms.value)
+                                          ctypes.byref(xp), ctypes.byref(ms)), "gpe_autotune")
+        return dict(tile_rows=ty.value, xcd_remap=rm.value, x_per_lane=xp.value, ms_per_sweep=ms.value)
+
+    def set_traversal(self, tile_rows, xcd_remap=1, x_per_lane=1):
+        self._check(self.lib.gpe_set_traversal(self._h, int(tile_rows), int(xcd_remap), int(x_per_lane)),
+                    "gpe_set_traversal")
+
+    def traversal(self):
+        ty, rm, xp = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.gpe_traversal(self._h, ctypes.byref(ty), ctypes.byref(rm), ctypes.byref(xp)),
+                    "gpe_traversal")
+        return dict(tile_rows=ty.value, xcd_remap=rm.value, x_per_lane=xp.value)
 
     def last_step_timing(self):
         ms = ctypes.c_double()

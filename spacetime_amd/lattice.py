@@ -97,6 +97,13 @@ class Lattice:
         sq = np.sqrt(self.vol)
         return (sq[:, None] * M) / sq[None, :]
 
+    def kinetic_bound(self, block_scale=1.0):
+        """Gershgorin bound on the spectrum of -1/2 L: max over unknowns of sum_f hodge_f / s^2
+        (diagonal sum_f hodge_f / 2s^2 plus as much off the diagonal).  Exact for BCC (6/s^2)."""
+        per = np.zeros(self.V)
+        np.add.at(per, self.face_src, self.face_hodge)
+        return float(per.max()) / (block_scale * block_scale)
+
     def packed_faces(self):
         """Faces packed as int32: (dx+1) | (dy+1) << 2 | (dz+1) << 4 | target << 8."""
         d = self.face_d + 1

@@ -111,8 +111,10 @@ def test_autotune_keeps_results_identical():
         st.start(gpe.START_EULER)
         t = st.autotune(5)
         assert t["tile_rows"] in (1, 2, 4, 8, 16, 24, 32) and t["xcd_remap"] in (0, 1) and t["ms_per_sweep"] > 0
+        assert t["x_per_lane"] in (1, 2)
         st.step(20)
         got = st.get_state(0)
         t2 = st.autotune(0)  # cached
-        assert (t2["tile_rows"], t2["xcd_remap"]) == (t["tile_rows"], t["xcd_remap"])
+        assert (t2["tile_rows"], t2["xcd_remap"], t2["x_per_lane"]) == (t["tile_rows"], t["xcd_remap"],
+                                                                          t["x_per_lane"])
     assert rel_l2(got, ref[0]) < 1e-6

@@ -22,7 +22,7 @@ def _config(lattice, ncomp, precision, dims, dt=2e-4, scheme=0, cell=0.6, **kw):
     return gpe.Config.centred_box(lattice, dims, cell, ncomp=ncomp, precision=precision, dt=dt, scheme=scheme, **phys)
 
 
-def _run_both(oracle, cfg, nsteps, field=None, seed=1):
+def _run_both(oracle, cfg, nsteps, field=None, seed=1, traversal=None):
     from spacetime_amd import lattice
     lat = lattice.get(cfg.lattice)
     L = oracle.lattice(lat)
@@ -36,6 +36,8 @@ def _run_both(oracle, cfg, nsteps, field=None, seed=1):
     with gpe.Stepper(cfg) as st:
         if field:
             st.set_field(**field)
+        if traversal:
+            st.set_traversal(*traversal)
         st.set_state(psi0)
         st.start(gpe.START_EULER)
         st.step(nsteps)
@@ -56,22 +58,43 @@ def test_fp64_sweeps_match_oracle(oracle, lattice_name, ncomp):
     assert rel_l2(g1, o1) < 1e-12
 
 
+DEGREE = {"cube": 6, "bcc": 4, "fcc": 8, "a15": 4, "c15": 4}
+
+
+@pytest.mark.parametrize("xpl", [1, 2])
 @pytest.mark.parametrize("lattice_name", ["cube", "bcc", "fcc", "a15", "c15"])
 @pytest.mark.parametrize("ncomp", [1, 3, 5])
-def test_fp32_sweeps_match_oracle(oracle, lattice_name, ncomp):
+def test_fp32_sweeps_match_oracle(oracle, lattice_name, ncomp, xpl):
+    """Both FP32 sweep bodies: one x per lane and x pairs per lane (odd nx: a half pair)."""
+    if xpl == 2 and DEGREE[lattice_name] * ncomp > 20:
+        pytest.skip("no pair-lane sweep for this degree x components")
     dims = (5, 4, 3) if lattice_name != "c15" else (3, 2, 2)
     cfg = _config(lattice_name, ncomp, gpe.FP32, dims)
-    (g0, g1), (o0, o1) = _run_both(oracle, cfg, 12, field=FIELD if ncomp > 1 else None)
+    (g0, g1), (o0, o1) = _run_both(oracle, cfg, 12, field=FIELD if ncomp > 1 else None, traversal=(8, 1, xpl))
     assert rel_l2(g0, o0) < 2e-5
     assert rel_l2(g1, o1) < 2e-5
 
 
-@pytest.mark.parametrize("precision,tol", [(1, 1e-12), (0, 2e-5)])
-def test_wide_rows_span_several_waves(oracle, precision, tol):
-    """nx > 64 exercises several wave segments per x-line and partial last waves."""
-    cfg = _config("bcc", 3, precision, (70, 3, 2))
-    (g0, g1), (o0, o1) = _run_both(oracle, cfg, 6, field=FIELD)
+@pytest.mark.parametrize("precision,tol,xpl,nx", [(1, 1e-12, 1, 70), (0, 2e-5, 1, 70), (0, 2e-5, 2, 70),
+                                                  (0, 2e-5, 2, 129), (0, 2e-5, 2, 128), (0, 2e-5, 1, 129)])
+def test_wide_rows_span_several_waves(oracle, precision, tol, xpl, nx):
+    """nx > 64 (or > 128 with x pairs) exercises several wave segments per x-line, partial last
+    waves, the line-end pair clamp (nx = 128: the last pair's +x neighbour pair starts at lx - 1)
+    and a lone last value (odd nx)."""
+    cfg = _config("bcc", 3, precision, (nx, 3, 2))
+    (g0, g1), (o0, o1) = _run_both(oracle, cfg, 6, field=FIELD, traversal=(2, 1, xpl))
     assert rel_l2(g0, o0) < tol and rel_l2(g1, o1) < tol
+
+
+def test_traversal_setter_validates():
+    cfg = _config("fcc", 5, gpe.FP32, (4, 4, 4))
+    with gpe.Stepper(cfg) as st:
+        with pytest.raises(gpe.GpeError, match="EINVAL"):
+            st.set_traversal(8, 1, 2)  # degree 8 x 5 components: no pair-lane sweep
+        with pytest.raises(gpe.GpeError, match="EINVAL"):
+            st.set_traversal(0, 1, 1)
+        st.set_traversal(4, 0, 1)
+        assert st.traversal() == dict(tile_rows=4, xcd_remap=0, x_per_lane=1)
 
 
 def test_loss_term(oracle):

@@ -104,3 +104,18 @@ def test_packed_faces_roundtrip():
         assert np.array_equal(((p >> 2) & 3) - 1, lat.face_d[:, 1])
         assert np.array_equal(((p >> 4) & 3) - 1, lat.face_d[:, 2])
         assert np.array_equal(p >> 8, lat.face_tgt)
+
+
+@pytest.mark.parametrize("name", ["cube", "bcc", "fcc", "a15", "c15"])
+def test_kinetic_bound_bounds_the_bloch_spectrum(name):
+    """Lattice.kinetic_bound (Gershgorin) is >= every Bloch eigenvalue of -1/2 L, and is the
+    exact maximum 6/s^2 on BCC (spectrum [0, 6/h^2])."""
+    L = lattice.get(name)
+    s = 0.37
+    bound = L.kinetic_bound(s)
+    lam = 0.0
+    for k in np.random.default_rng(3).uniform(-np.pi, np.pi, size=(60, 3)) / (s * L.block_width):
+        lam = max(lam, np.linalg.eigvalsh(L.laplacian_symbol(k, s)).max())
+    assert lam <= bound * (1 + 1e-12)
+    if name == "bcc":
+        assert bound == pytest.approx(6.0 / s ** 2)
