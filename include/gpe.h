@@ -210,6 +210,12 @@ int gpe_autotune(gpe_ctx *ctx, int trial_sweeps, int *tile_rows, int *xcd_remap,
 int gpe_set_traversal(gpe_ctx *ctx, int tile_rows, int xcd_remap, int x_per_lane);
 int gpe_traversal(const gpe_ctx *ctx, int *tile_rows, int *xcd_remap, int *x_per_lane);
 
+/* Achievable HBM rate on the current device, from streaming kernels over two buffers of
+ * `bytes` each (16-byte non-temporal accesses), averaged over `reps` launches: a copy
+ * (2 x bytes moved) and an in-place b += a / 2 (3 x bytes: the sweep's read-2-write-1 pattern).
+ * GB/s out; either output may be NULL.  No counterpart in the reference. */
+int gpe_stream_bandwidth(long long bytes, int reps, double *copy_gbs, double *update_gbs);
+
 /* ---------------------------------------------------------------- initial states ---- */
 /* Stationary vortex state f(r, z) e^{i kappa phi} of the scalar GPE in the trap
  * 0.5 (r^2 + ratio^2 z^2), by the reference's axisymmetric relaxation

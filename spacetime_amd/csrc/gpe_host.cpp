@@ -1318,6 +1318,40 @@ int gpe_autotune(gpe_ctx *c, int trial_sweeps, int *tile_rows, int *xcd_remap, i
     return GPE_OK;
 }
 
+int gpe_stream_bandwidth(long long bytes, int reps, double *copy_gbs, double *update_gbs)
+{
+    if (bytes < (1 << 20) || reps < 1) return GPE_EINVAL;
+    bytes = bytes / 16 * 16;
+    void *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = GPE_OK;
+    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) rc = GPE_ENOMEM;
+    if (!rc && (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+                hipEventCreate(&e1) != hipSuccess))
+        rc = GPE_EDEVICE;
+    if (!rc && (hipMemsetAsync(a, 0, bytes, s) != hipSuccess || hipMemsetAsync(b, 0, bytes, s) != hipSuccess))
+        rc = GPE_EDEVICE;
+    double *out[2] = {copy_gbs, update_gbs};
+    const double moved[2] = {2.0 * bytes, 3.0 * bytes};
+    for (int kind = 0; kind < 2 && !rc; ++kind) {
+        bool ok = launch_stream(kind, a, b, bytes / 16, s) == hipSuccess && hipEventRecord(e0, s) == hipSuccess;
+        for (int r = 0; r < reps && ok; ++r) ok = launch_stream(kind, a, b, bytes / 16, s) == hipSuccess;
+        ok = ok && hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+        float ms = 0.f;
+        ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+        if (!ok) rc = GPE_EDEVICE;
+        else if (out[kind]) *out[kind] = moved[kind] * reps / (ms * 1e-3) / 1e9;
+    }
+    if (s) (void)hipStreamSynchronize(s);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (s) (void)hipStreamDestroy(s);
+    return rc;
+}
+
 int gpe_last_step_timing(const gpe_ctx *c, double *ms_per_sweep, double *bytes_per_sweep)
 {
     if (!c) return GPE_EINVAL;

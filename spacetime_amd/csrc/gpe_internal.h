@@ -175,6 +175,9 @@ hipError_t launch_ensemble_sweep(int fp64, int ncomp, int deg, int xpl, const En
 hipError_t launch_advance_base(int *base, int by, hipStream_t s);
 hipError_t launch_set_base(int *base, int value, hipStream_t s);
 
+// kind 0: b = a; kind 1: b = b + a / 2; n16 16-byte elements.
+hipError_t launch_stream(int kind, const void *a, void *b, long long n16, hipStream_t s);
+
 // Device-side Thomas-Fermi fill with spinor weights and counter-based noise.
 struct FillArgs {
     double mu, g, amp;

@@ -759,6 +759,32 @@ hipError_t launch_local_fields(int fp64, int ncomp, const Geometry &g, const voi
     return hipGetLastError();
 }
 
+// Streaming reference kernels for the achievable HBM rate: a copy and an in-place
+// read-2-write-1 update (the sweep's own HBM pattern), 16-byte non-temporal accesses.
+typedef float f4s __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_stream_copy(const f4s *__restrict__ a, f4s *__restrict__ b, long long n)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) __builtin_nontemporal_store(__builtin_nontemporal_load(&a[i]), &b[i]);
+}
+
+__global__ __launch_bounds__(256) void k_stream_update(const f4s *__restrict__ a, f4s *b, long long n, float s)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) __builtin_nontemporal_store(__builtin_nontemporal_load(&b[i]) + s * a[i], &b[i]);
+}
+
+hipError_t launch_stream(int kind, const void *a, void *b, long long n16, hipStream_t s)
+{
+    const unsigned grid = (unsigned)((n16 + 255) / 256);
+    if (kind == 0)
+        hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, s, (const f4s *)a, (f4s *)b, n16);
+    else
+        hipLaunchKernelGGL(k_stream_update, dim3(grid), dim3(256), 0, s, (const f4s *)a, (f4s *)b, n16, 0.5f);
+    return hipGetLastError();
+}
+
 // ABI index (c, z, y, x, v) <-> device index (c, z, y, v, x)
 template <typename R, bool PACK>
 __global__ __launch_bounds__(256) void k_pack(Geometry g, int ncomp, const double2 *abi_in, double2 *abi_out,

@@ -119,6 +119,7 @@ def load_library(path=LIB_PATH):
     lib.gpe_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                  ctypes.POINTER(ctypes.c_int), _dp]
     lib.gpe_set_trap.argtypes = [vp, _dp]
+    lib.gpe_stream_bandwidth.argtypes = [ctypes.c_longlong, ctypes.c_int, _dp, _dp]
     lib.gpe_set_traversal.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.gpe_traversal.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_int)]
@@ -386,6 +387,17 @@ class Stepper:
         by = ctypes.c_double()
         self._check(self.lib.gpe_last_step_timing(self._h, ctypes.byref(ms), ctypes.byref(by)), "timing")
         return ms.value, by.value
+
+
+def stream_bandwidth(nbytes=1 << 30, reps=20):
+    """(copy GB/s, in-place read-2-write-1 GB/s) of 16-byte streaming kernels over two buffers of
+    nbytes each on the current device (gpe_stream_bandwidth)."""
+    lib = load_library()
+    c, u = ctypes.c_double(), ctypes.c_double()
+    rc = lib.gpe_stream_bandwidth(int(nbytes), int(reps), ctypes.byref(c), ctypes.byref(u))
+    if rc != 0:
+        raise GpeError("gpe_stream_bandwidth failed: %s (%d)" % (ERRORS.get(rc, "?"), rc))
+    return c.value, u.value
 
 
 class Ensemble:
