@@ -39,12 +39,12 @@ sys.path.insert(0, ROOT)
 METRIC = "node-updates/s on 1 GPU, FP32-form vs FP64 at ≤1e-4 rel L2; % HBM roofline"
 PEAK_HBM_GBS = 8000.0
 
-# HBM bytes per launch of the FP32 sweep kernel, measured with rocprofv3 --pmc FETCH_SIZE and
-# --pmc WRITE_SIZE on this workload (profiles/k_sweep_spin1_bcc_128.md); FETCH_SIZE is scaled by
-# 2.000 and WRITE_SIZE by 1.000, the ratios of bytes copied to counter readings for a 2 GiB
-# device copy measured in the same runs.
+# HBM bytes per launch of the FP32 sweep kernel (traversal 16 rows, XCD remap, two x per lane),
+# measured with rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE on this workload
+# (profiles/k_sweep_spin1_bcc_128.md); FETCH_SIZE is scaled by 2.000 and WRITE_SIZE by 1.000,
+# the ratios of bytes moved to counter readings for the 1 GiB streaming copy in the same runs.
 PMC_TRAFFIC = {
-    "spin1_bcc_128^3": 607131.6 * 1024 * 2.000 + 589824.0 * 1024 * 1.000,
+    "spin1_bcc_128^3": 599469.6 * 1024 * 2.000 + 589824.1 * 1024 * 1.000,
 }
 
 
