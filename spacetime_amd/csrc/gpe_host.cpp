@@ -85,6 +85,7 @@ struct gpe_ctx {
     // reductions
     double *d_partials = nullptr;
     double *d_scalars = nullptr;   // [0..15] results / scales
+    double *d_red = nullptr;       // FIN_SCRATCH doubles: first-stage results of launch_finalize
     // static field and ramp
     double bq = 0, bb[3] = {0, 0, 0}, bqq = 0, bbq[3] = {0, 0, 0};
     Ramp ramp;
@@ -376,6 +377,7 @@ int gpe_create(const gpe_config *cfg, gpe_ctx **out)
     if (!rc && hipMalloc((void **)&c->d_partials, sizeof(double) * OBS_COUNT * (size_t)g.blocks) != hipSuccess)
         rc = GPE_ENOMEM;
     if (!rc && hipMalloc((void **)&c->d_scalars, sizeof(double) * 64) != hipSuccess) rc = GPE_ENOMEM;
+    if (!rc && hipMalloc((void **)&c->d_red, sizeof(double) * FIN_SCRATCH) != hipSuccess) rc = GPE_ENOMEM;
     if (!rc) rc = ensure_fields(c, 1024);
     if (!rc && (e = hipEventCreate(&c->ev0)) != hipSuccess) rc = hip_fail(c, e, "hipEventCreate");
     if (!rc && (e = hipEventCreate(&c->ev1)) != hipSuccess) rc = hip_fail(c, e, "hipEventCreate");
@@ -395,7 +397,7 @@ int gpe_destroy(gpe_ctx *c)
     for (int l = 0; l < 2; ++l)
         if (c->level[l]) (void)hipFree(c->level[l]);
     void *ptrs[] = {c->d_face, c->d_w32, c->d_w64, c->d_wsum32, c->d_wsum64, c->d_pos32,
-                    c->d_pos64, c->d_dv, c->d_fields, c->d_partials, c->d_scalars};
+                    c->d_pos64, c->d_dv, c->d_fields, c->d_partials, c->d_scalars, c->d_red};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->h_fields) (void)hipHostFree(c->h_fields);
@@ -596,14 +598,14 @@ int gpe_itp(gpe_ctx *c, int nsteps, double dt_itp)
     oa.quad = quad_active(c);
     oa.partials = c->d_partials;
     hipError_t e = launch_observables(c->fp64, c->cfg.ncomp, oa, c->stream);
-    if (e == hipSuccess) e = launch_finalize(c->d_partials, c->g.blocks, OBS_COUNT, OBS_MAXDENS, c->d_scalars + 8, c->stream);
+    if (e == hipSuccess) e = launch_finalize(c->d_partials, c->g.blocks, OBS_COUNT, OBS_MAXDENS, c->d_scalars + 8, c->d_red, c->stream);
     if (e == hipSuccess) e = launch_norm_to_scale(c->d_scalars + 8 + OBS_NORM, 1.0, scale[0], c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "ITP initial norm");
     for (int i = 0; i < nsteps; ++i) {
         SweepArgs a = sweep_args(c, 0, c->level[0], c->level[1], dt_itp);
         a.scale_in = scale[i & 1];
         e = launch_sweep(c->fp64, c->cfg.ncomp, MODE_ITP, a, c->stream);
-        if (e == hipSuccess) e = launch_finalize(c->d_partials, c->gs.blocks, 1, -1, norm, c->stream);
+        if (e == hipSuccess) e = launch_finalize(c->d_partials, c->gs.blocks, 1, -1, norm, c->d_red, c->stream);
         if (e == hipSuccess) e = launch_norm_to_scale(norm, 1.0, scale[(i + 1) & 1], c->stream);
         if (e != hipSuccess) return hip_fail(c, e, "ITP step");
         std::swap(c->level[0], c->level[1]);
@@ -631,7 +633,7 @@ int gpe_observables(gpe_ctx *c, int level, gpe_observables_t *out)
     oa.quad = quad_active(c);
     oa.partials = c->d_partials;
     hipError_t e = launch_observables(c->fp64, c->cfg.ncomp, oa, c->stream);
-    if (e == hipSuccess) e = launch_finalize(c->d_partials, c->g.blocks, OBS_COUNT, OBS_MAXDENS, c->d_scalars + 8, c->stream);
+    if (e == hipSuccess) e = launch_finalize(c->d_partials, c->g.blocks, OBS_COUNT, OBS_MAXDENS, c->d_scalars + 8, c->d_red, c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "observables");
     double h[OBS_COUNT];
     HIPCALL(c, hipMemcpyAsync(h, c->d_scalars + 8, sizeof(h), hipMemcpyDeviceToHost, c->stream));
@@ -864,7 +866,7 @@ int gpe_adjoint(gpe_ctx *c, const gpe_adjoint_spec *spec, gpe_adjoint_result *re
         ADJ_CHECK(launch_pack(c->fp64, g, c->cfg.ncomp, staging, target, c->stream), "pack target");
         oa.mode = OBJ_PARTIALS;
         ADJ_CHECK(launch_objective(c->fp64, c->cfg.ncomp, oa, c->stream), "overlap");
-        ADJ_CHECK(launch_finalize(d_part, g.blocks, 2, -1, c->d_scalars + 32, c->stream), "overlap sum");
+        ADJ_CHECK(launch_finalize(d_part, g.blocks, 2, -1, c->d_scalars + 32, c->d_red, c->stream), "overlap sum");
         oa.mode = OBJ_OVERLAP;
         ADJ_CHECK(launch_objective(c->fp64, c->cfg.ncomp, oa, c->stream), "overlap grad");
         double ov[2];
@@ -883,7 +885,7 @@ int gpe_adjoint(gpe_ctx *c, const gpe_adjoint_spec *spec, gpe_adjoint_result *re
             for (int k = 0; k < 3; ++k) explicit_mag[k] = o.magnetization[k];
             oa.mode = OBJ_EXPLICIT;
             ADJ_CHECK(launch_objective(c->fp64, c->cfg.ncomp, oa, c->stream), "explicit terms");
-            ADJ_CHECK(launch_finalize(d_part, c->g.blocks, 4, -1, c->d_scalars + 40, c->stream), "explicit sums");
+            ADJ_CHECK(launch_finalize(d_part, c->g.blocks, 4, -1, c->d_scalars + 40, c->d_red, c->stream), "explicit sums");
             HIPCALL(c, hipMemcpyAsync(explicit_mom, c->d_scalars + 40, sizeof(explicit_mom), hipMemcpyDeviceToHost,
                                       c->stream));
             HIPCALL(c, hipStreamSynchronize(c->stream));
@@ -935,7 +937,8 @@ int gpe_adjoint(gpe_ctx *c, const gpe_adjoint_spec *spec, gpe_adjoint_result *re
             aa.coef = 2.0 * sdt;
             aa.partials = d_part;
             ADJ_CHECK(launch_adjoint(c->fp64, c->cfg.ncomp, aa, c->stream), "adjoint sweep");
-            ADJ_CHECK(launch_finalize(d_part, c->g.blocks, ADJ_COUNT, -1, d_steps + (size_t)ADJ_COUNT * k, c->stream),
+            ADJ_CHECK(launch_finalize(d_part, c->g.blocks, ADJ_COUNT, -1, d_steps + (size_t)ADJ_COUNT * k, c->d_red,
+                                      c->stream),
                       "adjoint sums");
             std::swap(A, B);
         }

@@ -191,7 +191,11 @@ hipError_t launch_fill_tf(int fp64, const Geometry &g, const Tables &tb, const P
 hipError_t launch_sweep(int fp64, int ncomp, int mode, const SweepArgs &a, hipStream_t s);
 hipError_t launch_observables(int fp64, int ncomp, const ObsArgs &a, hipStream_t s);
 // sum `n` partial records of `width` doubles into out[width] (max for column maxcol, if >= 0)
-hipError_t launch_finalize(const double *partials, int n, int width, int maxcol, double *out, hipStream_t s);
+// scratch: FIN_SCRATCH doubles of device memory owned by the caller's context.
+constexpr int FIN_WIDTH_MAX = 16;
+constexpr int FIN_SCRATCH = 256 * FIN_WIDTH_MAX;
+hipError_t launch_finalize(const double *partials, int n, int width, int maxcol, double *out, double *scratch,
+                           hipStream_t s);
 // out = sqrt(target / in[0])
 hipError_t launch_norm_to_scale(const double *in, double target, double *out, hipStream_t s);
 // level *= factor (device scalar if dfactor != nullptr)

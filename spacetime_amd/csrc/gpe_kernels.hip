@@ -540,37 +540,31 @@ __global__ __launch_bounds__(256) void k_observables(ObsArgs a)
 
 // --------------------------------------------------------------------- small kernels --
 
-__global__ __launch_bounds__(1024) void k_finalize(const double *partials, int n, int width, int maxcol, double *out)
+// Deterministic two-stage reduction of per-workgroup partials (n rows x width columns; column
+// maxcol takes the maximum, the others the sum): stage 1 gives each of up to FIN_BLOCKS
+// workgroups a fixed contiguous chunk of rows, stage 2 reduces the chunk results in one
+// workgroup.  Fixed chunks, strides and trees: the same inputs give the same bits.
+constexpr int FIN_BLOCKS = 256;
+
+__device__ __forceinline__ double fin_op(bool is_max, double a, double b) { return is_max ? fmax(a, b) : a + b; }
+
+__global__ __launch_bounds__(256) void k_finalize_stage(const double *partials, int n, int width, int maxcol,
+                                                        int chunk, double *out)
 {
-    // deterministic: fixed per-thread strides and a fixed tree
-    __shared__ double red[1024];
+    __shared__ double red[256];
+    const int lo = blockIdx.x * chunk;
+    const int hi = min(n, lo + chunk);
     for (int col = 0; col < width; ++col) {
         const bool is_max = col == maxcol;
-        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        int i = threadIdx.x;
-        for (; i + 7 * 1024 < n; i += 8 * 1024) {
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const double v = partials[(long long)(i + u * 1024) * width + col];
-                acc[u] = is_max ? fmax(acc[u], v) : acc[u] + v;
-            }
-        }
-        for (int u = 0; i < n; i += 1024, ++u) {
-            const double v = partials[(long long)i * width + col];
-            acc[u & 7] = is_max ? fmax(acc[u & 7], v) : acc[u & 7] + v;
-        }
-        double s = acc[0];
-#pragma unroll
-        for (int u = 1; u < 8; ++u) s = is_max ? fmax(s, acc[u]) : s + acc[u];
-        red[threadIdx.x] = s;
+        double acc = is_max ? -INFINITY : 0.0;
+        for (int i = lo + (int)threadIdx.x; i < hi; i += 256) acc = fin_op(is_max, acc, partials[(long long)i * width + col]);
+        red[threadIdx.x] = acc;
         __syncthreads();
-        for (int st = 512; st > 0; st >>= 1) {
-            if ((int)threadIdx.x < st)
-                red[threadIdx.x] = is_max ? fmax(red[threadIdx.x], red[threadIdx.x + st])
-                                          : red[threadIdx.x] + red[threadIdx.x + st];
+        for (int st = 128; st > 0; st >>= 1) {
+            if ((int)threadIdx.x < st) red[threadIdx.x] = fin_op(is_max, red[threadIdx.x], red[threadIdx.x + st]);
             __syncthreads();
         }
-        if (threadIdx.x == 0) out[col] = red[0];
+        if (threadIdx.x == 0) out[(long long)blockIdx.x * width + col] = red[0];
         __syncthreads();
     }
 }
@@ -956,9 +950,21 @@ hipError_t launch_set_base(int *base, int value, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_finalize(const double *partials, int n, int width, int maxcol, double *out, hipStream_t s)
+hipError_t launch_finalize(const double *partials, int n, int width, int maxcol, double *out, double *scratch,
+                           hipStream_t s)
 {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, partials, n, width, maxcol, out);
+    if (n <= 0 || width <= 0 || width > FIN_WIDTH_MAX) return hipErrorInvalidValue;
+    int nb = (n + 1023) / 1024;
+    if (nb > FIN_BLOCKS) nb = FIN_BLOCKS;
+    if (nb == 1) {
+        hipLaunchKernelGGL(k_finalize_stage, dim3(1), dim3(256), 0, s, partials, n, width, maxcol, n, out);
+        return hipGetLastError();
+    }
+    const int chunk = (n + nb - 1) / nb;
+    nb = (n + chunk - 1) / chunk;
+    hipLaunchKernelGGL(k_finalize_stage, dim3(nb), dim3(256), 0, s, partials, n, width, maxcol, chunk, scratch);
+    hipLaunchKernelGGL(k_finalize_stage, dim3(1), dim3(256), 0, s, (const double *)scratch, nb, width, maxcol, nb,
+                       out);
     return hipGetLastError();
 }
 
