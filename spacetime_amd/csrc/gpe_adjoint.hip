@@ -35,7 +35,7 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs a)
         const V2 *__restrict__ psi = (const V2 *)a.psi;
         const V2 *__restrict__ mu1 = (const V2 *)a.mu1;
         V2 *__restrict__ mu2 = (V2 *)a.mu2;
-        const long long cstride = (long long)g.nz * g.ny * g.V * g.lx;
+        const long long cstride = comp_stride(g);
         const long long own = line_index(g, 0, z, y, v);
 
         R pr[C], pi[C], mr[C], mi[C], qr[C], qi[C], ar[C], ai[C];
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(256) void k_objective(ObjArgs a)
     double p0 = 0, p1 = 0, p2 = 0, p3 = 0;
     if (a.mode == OBJ_EXPLICIT && wc.valid && wc.x < g.nx) {
         const int x = wc.x, v = wc.v, y = wc.y, z = wc.z;
-        const long long cstride = (long long)g.nz * g.ny * g.V * g.lx;
+        const long long cstride = comp_stride(g);
         const long long own = line_index(g, 0, z, y, v) + x;
         const double dv = a.tb.dv[v];
         const V2 *psi = (const V2 *)a.psi;
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(256) void k_objective(ObjArgs a)
     }
     if (a.mode != OBJ_EXPLICIT && wc.valid && wc.x < g.nx) {
         const int x = wc.x, v = wc.v, y = wc.y, z = wc.z;
-        const long long cstride = (long long)g.nz * g.ny * g.V * g.lx;
+        const long long cstride = comp_stride(g);
         const long long own = line_index(g, 0, z, y, v) + x;
         const double dv = a.tb.dv[v];
         const V2 *psi = (const V2 *)a.psi;

@@ -86,8 +86,11 @@ __device__ __forceinline__ WaveCoord decode(const Geometry &g) { return decode(g
 
 __device__ __forceinline__ long long line_index(const Geometry &g, int c, int z, int y, int v)
 {
-    return ((long long)((c * g.nz + z) * g.ny + y) * g.V + v) * g.lx;
+    return ((long long)((z * g.ny + y) * g.V + v) * g.C + c) * g.lx;
 }
+
+// Elements between the same x of consecutive components of one line.
+__device__ __forceinline__ long long comp_stride(const Geometry &g) { return g.lx; }
 
 // ------------------------------------------------------------------ local Hamiltonian --
 

@@ -75,7 +75,7 @@ struct gpe_ctx {
     int device = 0;
     void *level[2] = {nullptr, nullptr};
     // tables
-    int *d_face = nullptr;
+    int *d_face = nullptr, *d_nface = nullptr;
     float *d_w32 = nullptr, *d_wsum32 = nullptr, *d_pos32 = nullptr;
     double *d_w64 = nullptr, *d_wsum64 = nullptr, *d_pos64 = nullptr, *d_dv = nullptr;
     // fields per sweep
@@ -168,6 +168,7 @@ Tables tables(const gpe_ctx *c)
     t.pos32 = c->d_pos32;
     t.pos64 = c->d_pos64;
     t.dv = c->d_dv;
+    t.nface = c->d_nface;
     return t;
 }
 
@@ -295,6 +296,7 @@ int gpe_create(const gpe_config *cfg, gpe_ctx **out)
     g.nz = cfg->nz;
     g.V = c->lat->V;
     g.deg = c->lat->deg;
+    g.C = cfg->ncomp;
     g.xpl = 1;
     g.ty = cfg->ny < 8 ? cfg->ny : 8;
     const long long align = c->fp64 ? 8 : 16;  // 128-byte line starts
@@ -342,7 +344,11 @@ int gpe_create(const gpe_config *cfg, gpe_ctx **out)
     const double s = cfg->block_scale;
     std::vector<double> w64(V * deg), wsum64(V, 0.0), pos64(3 * V), dv(V);
     std::vector<float> w32(V * deg), wsum32(V), pos32(3 * V);
+    std::vector<int> nface(V);
     for (int v = 0; v < V; ++v) {
+        int nf = deg;  // pad faces (zero Hodge weight) trail the real ones
+        while (nf > 0 && c->lat->hodge[v * deg + nf - 1] == 0.0) --nf;
+        nface[v] = nf;
         for (int f = 0; f < deg; ++f) {
             const double w = -0.5 * c->lat->hodge[v * deg + f] / (s * s);
             w64[v * deg + f] = w;
@@ -367,6 +373,7 @@ int gpe_create(const gpe_config *cfg, gpe_ctx **out)
         if (ee != hipSuccess) rc = hip_fail(c, ee, "hipMemcpy tables");
     };
     up((void **)&c->d_face, c->lat->face, sizeof(int) * V * deg);
+    up((void **)&c->d_nface, nface.data(), sizeof(int) * V);
     up((void **)&c->d_w32, w32.data(), sizeof(float) * V * deg);
     up((void **)&c->d_w64, w64.data(), sizeof(double) * V * deg);
     up((void **)&c->d_wsum32, wsum32.data(), sizeof(float) * V);
@@ -396,7 +403,7 @@ int gpe_destroy(gpe_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int l = 0; l < 2; ++l)
         if (c->level[l]) (void)hipFree(c->level[l]);
-    void *ptrs[] = {c->d_face, c->d_w32, c->d_w64, c->d_wsum32, c->d_wsum64, c->d_pos32,
+    void *ptrs[] = {c->d_face, c->d_nface, c->d_w32, c->d_w64, c->d_wsum32, c->d_wsum64, c->d_pos32,
                     c->d_pos64, c->d_dv, c->d_fields, c->d_partials, c->d_scalars, c->d_red};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);

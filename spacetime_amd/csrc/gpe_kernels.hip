@@ -35,7 +35,7 @@ __device__ __forceinline__ void sweep_body_single(const SweepArgs &a, int sweep,
         const V2 *__restrict__ cur = (const V2 *)a.cur;
         V2 *__restrict__ nxt = (V2 *)a.next;
         const V2 *__restrict__ prv = (const V2 *)(a.prev ? a.prev : a.next);
-        const long long cstride = (long long)g.nz * g.ny * g.V * g.lx;  // elements per component
+        const long long cstride = comp_stride(g);  // elements between components of a line
         const long long own_line = line_index(g, 0, z, y, v);
 
         // issue every load of the wave before any use: own values, psi_{k-1}, and the
@@ -78,11 +78,20 @@ __device__ __forceinline__ void sweep_body_single(const SweepArgs &a, int sweep,
             wf[f] = (inside && xok) ? wt[f] : (R)0;
             nbase[f] = cur + line_index(g, 0, zn, yn, t);
         }
+        // degree-8 tables pad some unknowns with zero-weight faces: skip their loads
+        const int nf = DEG == 8 ? a.tb.nface[v] : DEG;
         V2 nb[DEG][C];
 #pragma unroll
         for (int f = 0; f < DEG; ++f)
 #pragma unroll
-            for (int c = 0; c < C; ++c) nb[f][c] = nbase[f][c * cstride + noff[f]];
+            for (int c = 0; c < C; ++c) {
+                if (f < nf) {
+                    nb[f][c] = nbase[f][c * cstride + noff[f]];
+                } else {
+                    nb[f][c].x = 0;
+                    nb[f][c].y = 0;
+                }
+            }
         // stencil: sum_f w_f psi_t(f); out-of-box neighbours carry weight 0
 #pragma unroll
         for (int f = 0; f < DEG; ++f)
@@ -186,7 +195,7 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
         const float2 *__restrict__ cur = (const float2 *)a.cur;
         float2 *__restrict__ nxt = (float2 *)a.next;
         const float2 *__restrict__ prv = (const float2 *)(a.prev ? a.prev : a.next);
-        const long long cstride = (long long)g.nz * g.ny * g.V * g.lx;
+        const long long cstride = comp_stride(g);
         const long long own_line = line_index(g, 0, z, y, v);
         const int lxm2 = (int)g.lx - 2;
 
@@ -233,11 +242,17 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
             ok1[f] = inside && (unsigned)(x0 + 1 + dx) < (unsigned)g.nx;
             nbase[f] = cur + line_index(g, 0, zn, yn, t) + sc;
         }
+        const int nf = DEG == 8 ? a.tb.nface[v] : DEG;  // skip zero-weight pad faces
         f4u nbv[DEG][C];
 #pragma unroll
         for (int f = 0; f < DEG; ++f)
 #pragma unroll
-            for (int c = 0; c < C; ++c) nbv[f][c] = *(const f4u *)(nbase[f] + c * cstride);
+            for (int c = 0; c < C; ++c) {
+                if (f < nf)
+                    nbv[f][c] = *(const f4u *)(nbase[f] + c * cstride);
+                else
+                    nbv[f][c] = (f4u)(0.f);
+            }
 #pragma unroll
         for (int f = 0; f < DEG; ++f) {
             const float w = wt[f];
