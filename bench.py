@@ -410,6 +410,46 @@ def cpu_baseline(cfg, psi0, log, budget_s=15.0):
                     str(d) for d in cfg.dims), nodes))
 
 
+def cpu_reference(log, iterations=3):
+    """The reference's own scalar CPU solver, timed: GrossPitaevskii/GPBlocks.cpp's
+    integrateInTime compiled from the reference sources into oracle/_ref (single thread, FP64
+    staggered steps), on its CPU-sized configuration: BCC blocks of edge 0.25 over [-4, 4]^3
+    (32^3 blocks, 393,216 unknowns), g = 300, vortex-free relaxed initial state.  node-updates/s
+    from the wall time between its per-iteration output lines (2 x steps x unknowns each)."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "oracle", "_ref", "gpblocks_driver")
+    if not os.path.exists(exe):
+        log("cpu reference: %s not built" % exe)
+        return None
+    args = [exe, "0", "300", "4", "4", "0.05", "0.05", "2000", "%.17g" % (0.25 / 8.0 ** 0.5), "1", str(iterations)]
+    grid, steps, stamps = None, None, []
+    with tempfile.TemporaryDirectory() as d:  # the solver writes a picture per iteration into its cwd
+        proc = subprocess.Popen(args, cwd=d, stdout=subprocess.PIPE, text=True, bufsize=1)
+        for line in proc.stdout:
+            now = time.perf_counter()
+            if line.startswith("GRID "):
+                grid = [int(v) for v in line.split()[1:5]]
+            elif line.startswith("STEPS_PER_ITERATION "):
+                steps = int(line.split()[1])
+            elif line.startswith("normsq="):
+                stamps.append(now)
+        proc.wait()
+    if proc.returncode != 0 or not grid or not steps or len(stamps) < 2:
+        log("cpu reference: run failed (rc %s)" % proc.returncode)
+        return None
+    unknowns = grid[0] * grid[1] * grid[2] * grid[3]
+    per_iter = min(b - a for a, b in zip(stamps, stamps[1:]))
+    val = 2.0 * steps * unknowns / per_iter
+    log("cpu reference (GPBlocks.cpp, 1 thread): %d unknowns, %d staggered steps per iteration in %.2fs: "
+        "%.3e node-updates/s" % (unknowns, steps, per_iter, val))
+    return dict(value=val, unit="node-updates/s", cores=1, kind="reference",
+                sample="GrossPitaevskii/GPBlocks.cpp integrateInTime compiled from the reference sources, "
+                       "BCC %dx%dx%d blocks (%d unknowns), g = 300, one iteration of %d staggered steps "
+                       "(%d sweeps), fastest of %d" % (grid[0], grid[1], grid[2], unknowns, steps, 2 * steps,
+                                                       len(stamps) - 1))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -472,9 +512,12 @@ def main():
         acc.update(norm_drift_fp64=r64["norm_drift"], energy_drift_fp64=r64["energy_drift"])
         acc["speedup_fp32_over_fp64"] = r32["node_updates_per_s"] / r64["node_updates_per_s"]
     del psi32
-    cpu = None
+    cpu, cpu_ref = None, None
     if not args.no_cpu:
         cpu = cpu_baseline(cfg, psi0, log)
+        cpu_ref = cpu_reference(log)
+        if cfg.ncomp == 1 and cpu_ref:  # the reference has a CPU path for the scalar GPE only
+            cpu, cpu_ref = cpu_ref, cpu
 
     line = {
         "metric": METRIC,
@@ -505,6 +548,7 @@ def main():
             "algorithmic_bytes_per_sweep": r32["bytes_per_sweep"],
         },
         "cpu_baseline": cpu,
+        "cpu_other": cpu_ref,
         "fp64_form": None if r64 is None else {
             "value": r64["node_updates_per_s"], "ms_per_step": r64["ms_per_step"],
             "achieved_gbs": r64["achieved_gbs"], "frac": r64["achieved_gbs"] / PEAK_HBM_GBS},

@@ -74,6 +74,13 @@ typedef struct {
     double quad_sign;     /* H += quad_sign (Bquad . f)^2; reference +1 spin-1, -1 spin-2 */
     double q;             /* constant quadratic Zeeman H += q f_z^2                    */
     double center[3];     /* quadrupole field centre (Spin1GpeBCC.cu:124-137)          */
+    /* Expanding grid after the trap is switched off (Spin1GpeBCC.cu:1579-1582, Spin2GpeBCC.cu:
+     * 1523-1540, where expansionBlockScale grows by k * block_scale per ms): the block scale used
+     * by the Laplacian and the positions is block_scale * sigma(t), sigma = 1 + expansion_rate *
+     * max(0, t - expansion_start) (rate = k * ms per time unit); the dual volumes of the
+     * integrals stay those of block_scale, as in the reference.  expansion_rate = 0: static. */
+    double expansion_start;
+    double expansion_rate;
 } gpe_config;
 
 /* Piecewise field ramp (getSignal, Spin1GpeBCC/AliceRingRamps.h:74-140 and

@@ -127,6 +127,9 @@ FieldRec field_at(const gpe_ctx *c, double t)
 {
     FieldRec f{};
     f.t = t;
+    f.sigma = 1.0;
+    if (c->cfg.expansion_rate != 0.0 && t > c->cfg.expansion_start)
+        f.sigma = 1.0 + c->cfg.expansion_rate * (t - c->cfg.expansion_start);
     if (c->ramp.active) {
         const Ramp &r = c->ramp;
         double sq, sb[3];
@@ -283,6 +286,7 @@ int gpe_create(const gpe_config *cfg, gpe_ctx **out)
     if (cfg->scheme != GPE_SCHEME_LEAPFROG3 && cfg->scheme != GPE_SCHEME_STAGGERED) return GPE_EINVAL;
     if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1) return GPE_EINVAL;
     if (!(cfg->block_scale > 0.0) || !(cfg->dt > 0.0)) return GPE_EINVAL;
+    if (!(cfg->expansion_rate >= 0.0)) return GPE_EINVAL;
     gpe_ctx *c = new (std::nothrow) gpe_ctx();
     if (!c) return GPE_ENOMEM;
     c->cfg = *cfg;
@@ -765,6 +769,7 @@ int gpe_adjoint(gpe_ctx *c, const gpe_adjoint_spec *spec, gpe_adjoint_result *re
     if (!spec || !res) return fail(c, GPE_EINVAL, "null adjoint spec or result");
     if (c->cfg.scheme != GPE_SCHEME_LEAPFROG3) return fail(c, GPE_EINVAL, "adjoint needs the 3-level scheme");
     if (c->cfg.alpha != 0.0) return fail(c, GPE_EINVAL, "adjoint needs alpha = 0");
+    if (c->cfg.expansion_rate != 0.0) return fail(c, GPE_EINVAL, "adjoint needs a static grid (expansion_rate = 0)");
     if (c->ramp.active && (c->ramp.qscale_q != 0.0 || c->ramp.qscale_b != 0.0))
         return fail(c, GPE_EINVAL, "adjoint does not differentiate a ramped quadratic field");
     const int N = spec->nsweeps, K = spec->checkpoint_every;

@@ -85,7 +85,7 @@ struct FieldRec {
     double bq, bb[3];
     double bqq, bbq[3];
     double t;
-    double pad;
+    double sigma;         // block-scale factor of the expanding grid (1: static)
 };
 
 enum Mode { MODE_UPDATE = 0, MODE_EULER = 1, MODE_ITP = 2, MODE_HPSI = 3 };

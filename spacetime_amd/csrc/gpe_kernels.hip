@@ -101,24 +101,26 @@ __device__ __forceinline__ void sweep_body_single(const SweepArgs &a, int sweep,
                 ai[c] += wf[f] * nb[f][c].y;
             }
         const R wsum = table_wsum<R>(a.tb)[v];
+        const Physics &ph = a.ph;
+        const FieldRec &fr = a.fields[sweep];
         R hr[C], hi[C];
         R sc = (R)1;
         if constexpr (MODE == MODE_ITP) sc = (R)(*a.scale_in);
+        const R sig = (R)fr.sigma;                          // expanding grid: lengths x sigma
+        const R ksc = sc * (R)(1.0 / (fr.sigma * fr.sigma));  // Laplacian / sigma^2
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-            hr[c] = sc * (ar[c] - wsum * pr[c]);
-            hi[c] = sc * (ai[c] - wsum * pi[c]);
+            hr[c] = ksc * (ar[c] - wsum * pr[c]);
+            hi[c] = ksc * (ai[c] - wsum * pi[c]);
             pr[c] *= sc;
             pi[c] *= sc;
         }
         // position, trap and field
-        const Physics &ph = a.ph;
-        const FieldRec &fr = a.fields[sweep];
         const R *pos = table_pos<R>(a.tb) + 3 * v;
         const R cell = (R)ph.cell;
-        const R rx = (R)ph.p0[0] + cell * (R)x + pos[0];
-        const R ry = (R)ph.p0[1] + cell * (R)y + pos[1];
-        const R rz = (R)ph.p0[2] + cell * (R)z + pos[2];
+        const R rx = (R)ph.p0[0] + sig * (cell * (R)x + pos[0]);
+        const R ry = (R)ph.p0[1] + sig * (cell * (R)y + pos[1]);
+        const R rz = (R)ph.p0[2] + sig * (cell * (R)z + pos[2]);
         R vtrap = 0;
         if (fr.t < ph.trap_off_time)
             vtrap = (R)0.5 * ((R)ph.trap2[0] * rx * rx + (R)ph.trap2[1] * ry * ry + (R)ph.trap2[2] * rz * rz) +
@@ -277,10 +279,12 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
         if constexpr (MODE == MODE_ITP) sc = (float)(*a.scale_in);
         const Physics &ph = a.ph;
         const FieldRec &fr = a.fields[sweep];
+        const float sig = (float)fr.sigma;                            // expanding grid: lengths x sigma
+        const float ksc = sc * (float)(1.0 / (fr.sigma * fr.sigma));  // Laplacian / sigma^2
         const float *pos = a.tb.pos32 + 3 * v;
         const float cell = (float)ph.cell;
-        const float ry = (float)ph.p0[1] + cell * (float)y + pos[1];
-        const float rz = (float)ph.p0[2] + cell * (float)z + pos[2];
+        const float ry = (float)ph.p0[1] + sig * (cell * (float)y + pos[1]);
+        const float rz = (float)ph.p0[2] + sig * (cell * (float)z + pos[2]);
         const bool trap_on = fr.t < ph.trap_off_time;
         const float coef = (float)a.coef;
         float2 out[2][C];
@@ -289,12 +293,12 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
             float hr[C], hi[C];
 #pragma unroll
             for (int c = 0; c < C; ++c) {
-                hr[c] = sc * (ar[e][c] - wsum * pr[e][c]);
-                hi[c] = sc * (ai[e][c] - wsum * pi[e][c]);
+                hr[c] = ksc * (ar[e][c] - wsum * pr[e][c]);
+                hi[c] = ksc * (ai[e][c] - wsum * pi[e][c]);
                 pr[e][c] *= sc;
                 pi[e][c] *= sc;
             }
-            const float rx = (float)ph.p0[0] + cell * (float)(x0 + e) + pos[0];
+            const float rx = (float)ph.p0[0] + sig * (cell * (float)(x0 + e) + pos[0]);
             float vtrap = 0.f;
             if (trap_on)
                 vtrap = 0.5f * ((float)ph.trap2[0] * rx * rx + (float)ph.trap2[1] * ry * ry +
@@ -463,9 +467,15 @@ __global__ __launch_bounds__(256) void k_observables(ObsArgs a)
         }
         const Physics &ph = a.ph;
         const double *pos = a.tb.pos64 + 3 * v;
-        const double rx = ph.p0[0] + ph.cell * x + pos[0];
-        const double ry = ph.p0[1] + ph.cell * y + pos[1];
-        const double rz = ph.p0[2] + ph.cell * z + pos[2];
+        const double sig = a.field.sigma, ik = 1.0 / (sig * sig);
+        const double rx = ph.p0[0] + sig * (ph.cell * x + pos[0]);
+        const double ry = ph.p0[1] + sig * (ph.cell * y + pos[1]);
+        const double rz = ph.p0[2] + sig * (ph.cell * z + pos[2]);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            kr[c] *= ik;
+            ki[c] *= ik;
+        }
         double vtrap = 0;
         if (a.field.t < ph.trap_off_time)
             vtrap = 0.5 * (ph.trap2[0] * rx * rx + ph.trap2[1] * ry * ry + ph.trap2[2] * rz * rz) + ph.trap_offset;

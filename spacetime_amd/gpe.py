@@ -39,6 +39,7 @@ class _Config(ctypes.Structure):
         ("trap", ctypes.c_double * 3), ("trap_offset", ctypes.c_double), ("trap_off_time", ctypes.c_double),
         ("c0", ctypes.c_double), ("c1", ctypes.c_double), ("c2", ctypes.c_double), ("alpha", ctypes.c_double),
         ("quad_sign", ctypes.c_double), ("q", ctypes.c_double), ("center", ctypes.c_double * 3),
+        ("expansion_start", ctypes.c_double), ("expansion_rate", ctypes.c_double),
     ]
 
 
@@ -152,6 +153,8 @@ class Config:
     quad_sign: float = None
     q: float = 0.0
     center: Sequence[float] = (0.0, 0.0, 0.0)
+    expansion_start: float = 1e300
+    expansion_rate: float = 0.0
 
     @property
     def V(self):
@@ -174,6 +177,7 @@ class Config:
         c.c0, c.c1, c.c2, c.alpha, c.q = self.c0, self.c1, self.c2, self.alpha, self.q
         c.quad_sign = (1.0 if self.ncomp == 3 else -1.0) if self.quad_sign is None else self.quad_sign
         c.center[:] = list(self.center)
+        c.expansion_start, c.expansion_rate = self.expansion_start, self.expansion_rate
         return c
 
     @classmethod

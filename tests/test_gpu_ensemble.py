@@ -118,3 +118,32 @@ def test_autotune_keeps_results_identical():
         assert (t2["tile_rows"], t2["xcd_remap"], t2["x_per_lane"]) == (t["tile_rows"], t["xcd_remap"],
                                                                           t["x_per_lane"])
     assert rel_l2(got, ref[0]) < 1e-6
+
+
+def test_graph_replays_each_match_eager():
+    """The captured K-sweep graph replayed several times, with eager work (state reads,
+    observables) and synchronisations between replays: every replay's result equals the same
+    member stepped eagerly on its own."""
+    mem = _members()[:3]
+    steppers, singles = [], []
+    for cfg, psi, ramp in mem:
+        st = gpe.Stepper(cfg)
+        st.set_ramp(q_segments=[(0.01, 1e30, 0)], b_segments=ramp)
+        st.set_state(psi)
+        st.start(gpe.START_EULER)
+        steppers.append(st)
+        ref = gpe.Stepper(cfg)
+        ref.set_ramp(q_segments=[(0.01, 1e30, 0)], b_segments=ramp)
+        ref.set_state(psi)
+        ref.start(gpe.START_EULER)
+        singles.append(ref)
+    with gpe.Ensemble(steppers) as ens:
+        for replay in range(4):
+            ens.step(6, graph_sweeps=6)
+            for st, ref in zip(steppers, singles):
+                ref.step(6)
+                st.observables(0, check=False)  # eager work between replays
+                assert rel_l2(st.get_state(0), ref.get_state(0)) < 1e-6, replay
+                assert st.time() == ref.time()
+    for s in steppers + singles:
+        s.close()

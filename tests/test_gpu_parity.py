@@ -271,3 +271,40 @@ def test_device_thomas_fermi_fill(lattice_name, ncomp, precision):
     assert rel_l2(clean, w[:, None, None, None, None] * tf[None]) < tol
     assert rel_l2(noisy, expect) < tol
     assert 0.0 < rel_l2(noisy, clean) < amp * np.sqrt(ncomp)
+
+
+@pytest.mark.parametrize("precision,tol", [(gpe.FP64, 1e-12), (gpe.FP32, 2e-5)])
+def test_expanding_grid_matches_oracle(oracle, precision, tol):
+    """Grid expansion after trap-off: the block scale of the Laplacian and positions is
+    s (1 + rate max(0, t - start)); the oracle is stepped sweep by sweep with that scale."""
+    from spacetime_amd import lattice
+    dt = 2e-4
+    cfg = _config("bcc", 3, precision, (6, 4, 3), dt=dt, expansion_start=3.5 * dt, expansion_rate=60.0,
+                  trap_off_time=3.5 * dt)
+    L = oracle.lattice(lattice.get("bcc"))
+    psi0 = random_state(cfg.shape, 8)
+    s0 = cfg.block_scale
+    p = with_field(oracle_params(oracle, cfg), **FIELD)
+
+    def at(t):
+        p.block_scale = s0 * (1.0 + cfg.expansion_rate * max(0.0, t - cfg.expansion_start))
+        return p
+
+    n = 14
+    a = psi0.copy()
+    b = oracle.forward_euler(L, at(0.0), a, dt)
+    for k in range(1, n + 1):
+        nxt = a.copy()
+        oracle.update(L, at(k * dt), nxt, b, 2 * dt, t=k * dt)
+        a, b = b, nxt
+    with gpe.Stepper(cfg) as st:
+        st.set_field(**FIELD)
+        st.set_state(psi0)
+        st.start(gpe.START_EULER)
+        st.step(n)
+        got = st.get_state(0)
+        with pytest.raises(gpe.GpeError, match="EINVAL"):
+            st.set_state(psi0)
+            st.adjoint(4, 2, gpe.OBJ_MAGNETIZATION_Z)
+    assert p.block_scale > 1.05 * s0
+    assert rel_l2(got, b) < tol
