@@ -1040,6 +1040,11 @@ struct gpe_ensemble {
     int *d_base = nullptr;
     int ncomp = 0, fp64 = 0, per = 1;
     hipStream_t stream = nullptr;
+    // graph replays run each group's sweeps as its own chain on one of these streams, forked
+    // from and joined back to `stream`, so independent groups overlap (default priority)
+    std::vector<hipStream_t> aux;
+    std::vector<hipEvent_t> join;
+    hipEvent_t fork = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     int graph_sweeps = 0;
@@ -1097,6 +1102,16 @@ int gpe_ensemble_create(gpe_ctx *const *members, int n, gpe_ensemble **out)
     if (!rc && hipMalloc((void **)&e->d, sizeof(EnsembleMember) * n) != hipSuccess) rc = GPE_ENOMEM;
     if (!rc && hipMalloc((void **)&e->d_base, sizeof(int) * 4) != hipSuccess) rc = GPE_ENOMEM;
     if (!rc && (hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess)) rc = GPE_EDEVICE;
+    const int naux = e->groups.size() > 1 ? std::min<int>((int)e->groups.size(), 4) : 0;
+    if (!rc && naux && hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess) rc = GPE_EDEVICE;
+    for (int i = 0; i < naux && !rc; ++i) {
+        hipStream_t st = nullptr;
+        hipEvent_t ev = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) rc = GPE_EDEVICE;
+        else e->aux.push_back(st);
+        if (!rc && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) rc = GPE_EDEVICE;
+        else if (!rc) e->join.push_back(ev);
+    }
     if (rc) {
         gpe_ensemble_destroy(e);
         return rc;
@@ -1115,6 +1130,12 @@ int gpe_ensemble_destroy(gpe_ensemble *e)
     if (e->d_base) (void)hipFree(e->d_base);
     if (e->ev0) (void)hipEventDestroy(e->ev0);
     if (e->ev1) (void)hipEventDestroy(e->ev1);
+    for (hipStream_t st : e->aux) {
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+    }
+    for (hipEvent_t ev : e->join) (void)hipEventDestroy(ev);
+    if (e->fork) (void)hipEventDestroy(e->fork);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return GPE_OK;
@@ -1151,19 +1172,40 @@ int gpe_ensemble_step(gpe_ensemble *e, int nsteps, int graph_sweeps)
     }
     ENSCALL(e, hipMemcpyAsync(e->d, e->h.data(), sizeof(EnsembleMember) * e->h.size(), hipMemcpyHostToDevice,
                               e->stream));
-    // one launch per group per sweep; the groups of a sweep are independent
+    // one launch per group per sweep; groups are independent of each other
+    auto launch_group = [&](const EnsembleGroup &gr, int k, const int *base, hipStream_t st) -> hipError_t {
+        EnsembleArgs a;
+        a.members = e->d + gr.first;
+        a.nmember = gr.count;
+        a.parity = k & 1;
+        a.sweep = k;
+        a.sweep_base = base;
+        return launch_ensemble_sweep(e->fp64, e->ncomp, gr.deg, gr.xpl, a, gr.blocks, st);
+    };
     auto launch_all = [&](int k, const int *base) -> hipError_t {
         for (const EnsembleGroup &gr : e->groups) {
-            EnsembleArgs a;
-            a.members = e->d + gr.first;
-            a.nmember = gr.count;
-            a.parity = k & 1;
-            a.sweep = k;
-            a.sweep_base = base;
-            hipError_t err = launch_ensemble_sweep(e->fp64, e->ncomp, gr.deg, gr.xpl, a, gr.blocks, e->stream);
+            hipError_t err = launch_group(gr, k, base, e->stream);
             if (err != hipSuccess) return err;
         }
         return hipSuccess;
+    };
+    // K sweeps of every group, each group a chain on a forked stream when there are several
+    auto capture_chains = [&](int K) -> hipError_t {
+        const size_t na = e->aux.size();
+        hipError_t err = hipSuccess;
+        if (na) {
+            err = hipEventRecord(e->fork, e->stream);
+            for (size_t i = 0; i < na && err == hipSuccess; ++i) err = hipStreamWaitEvent(e->aux[i], e->fork, 0);
+        }
+        for (size_t g = 0; g < e->groups.size() && err == hipSuccess; ++g) {
+            hipStream_t st = na ? e->aux[g % na] : e->stream;
+            for (int k = 0; k < K && err == hipSuccess; ++k) err = launch_group(e->groups[g], k, e->d_base, st);
+        }
+        for (size_t i = 0; i < na && err == hipSuccess; ++i) {
+            err = hipEventRecord(e->join[i], e->aux[i]);
+            if (err == hipSuccess) err = hipStreamWaitEvent(e->stream, e->join[i], 0);
+        }
+        return err;
     };
     ENSCALL(e, hipEventRecord(e->ev0, e->stream));
     if (graph_sweeps > 0) {
@@ -1173,8 +1215,7 @@ int gpe_ensemble_step(gpe_ensemble *e, int nsteps, int graph_sweeps)
             e->gexec = nullptr;
             e->graph = nullptr;
             ENSCALL(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-            hipError_t cap = hipSuccess;
-            for (int k = 0; k < graph_sweeps && cap == hipSuccess; ++k) cap = launch_all(k, e->d_base);
+            hipError_t cap = capture_chains(graph_sweeps);
             if (cap == hipSuccess) cap = launch_advance_base(e->d_base, graph_sweeps, e->stream);
             hipGraph_t captured = nullptr;
             const hipError_t end = hipStreamEndCapture(e->stream, &captured);
