@@ -203,7 +203,7 @@ int gpe_ensemble_timing(const gpe_ensemble *ens, double *ms_per_sweep, double *u
 
 /* -------------------------------------------------------------------- autotuning ---- */
 /* Time the traversal candidates (y-tile rows 1..32 x XCD remap on/off x one or two x values
- * per lane, the latter for FP32 where the kernel has it) on scratch copies of the context's
+ * per lane, the latter for FP32) on scratch copies of the context's
  * two levels for trial_sweeps sweeps each, and keep the fastest.  Results are cached per
  * (lattice, ncomp, precision, nx, ny, nz) for the process; pass trial_sweeps = 0 to apply a
  * cached choice only.  Outputs may be NULL.  No counterpart in the reference, which fixes its
@@ -212,7 +212,7 @@ int gpe_autotune(gpe_ctx *ctx, int trial_sweeps, int *tile_rows, int *xcd_remap,
                  double *ms_per_sweep);
 
 /* Set / read the sweep traversal directly: y-tile rows (1..64), XCD remap (0/1), x values per
- * lane (1, or 2 where gpe_autotune would consider it).  Results do not depend on it beyond
+ * lane (1, or 2 for FP32).  Results do not depend on it beyond
  * FP32 instruction selection. */
 int gpe_set_traversal(gpe_ctx *ctx, int tile_rows, int xcd_remap, int x_per_lane);
 int gpe_traversal(const gpe_ctx *ctx, int *tile_rows, int *xcd_remap, int *x_per_lane);

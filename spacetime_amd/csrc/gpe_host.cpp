@@ -1283,7 +1283,7 @@ int gpe_set_traversal(gpe_ctx *c, int tile_rows, int xcd_remap, int x_per_lane)
     if (tile_rows < 1 || tile_rows > 64 || (xcd_remap != 0 && xcd_remap != 1) || (x_per_lane != 1 && x_per_lane != 2))
         return fail(c, GPE_EINVAL, "bad traversal");
     if (x_per_lane == 2 && !pair_lanes_ok(c->fp64, c->g.deg, c->cfg.ncomp))
-        return fail(c, GPE_EINVAL, "two x per lane needs FP32 and degree x components <= 20");
+        return fail(c, GPE_EINVAL, "two x per lane needs FP32");
     const int before = c->g.blocks;
     set_traversal(c, tile_rows, xcd_remap, x_per_lane);
     return grow_partials(c, before);

@@ -41,19 +41,20 @@ struct Geometry {
 };
 
 // x values per lane of the sweep kernel.  Two (one 16-byte load per component and neighbour,
-// the access width of the FP64 form) exist for FP32 where the DEG * C neighbour pairs fit the
-// register budget; which one runs is a traversal choice (default_xpl, then the autotuner).
+// the access width of the FP64 form) exist for FP32; which one runs is a traversal choice
+// (default_xpl, then the autotuner).
 __host__ __device__ constexpr bool pair_lanes_ok(int fp64, int deg, int ncomp)
 {
-    return !fp64 && deg * ncomp <= 20;
+    return !fp64 && deg >= 1 && ncomp >= 1;
 }
 
-// Default: pairs when allowed and they leave no more idle lanes than single values do
+// Default: pairs where the DEG * C neighbour pairs fit the register budget without cutting
+// occupancy below 4 waves per SIMD, and they leave no more idle lanes than single values do
 // (x-lines of 48 or 64 would idle half of each wave), or for one component, where the
 // per-wave overhead of single values dominates.
 __host__ __device__ constexpr int default_xpl(int fp64, int deg, int ncomp, int nx)
 {
-    return (pair_lanes_ok(fp64, deg, ncomp) &&
+    return (pair_lanes_ok(fp64, deg, ncomp) && deg * ncomp <= 20 &&
             (ncomp == 1 || (nx + 127) / 128 * 128 <= (nx + 63) / 64 * 64)) ? 2 : 1;
 }
 

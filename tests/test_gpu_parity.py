@@ -58,16 +58,11 @@ def test_fp64_sweeps_match_oracle(oracle, lattice_name, ncomp):
     assert rel_l2(g1, o1) < 1e-12
 
 
-DEGREE = {"cube": 6, "bcc": 4, "fcc": 8, "a15": 4, "c15": 4}
-
-
 @pytest.mark.parametrize("xpl", [1, 2])
 @pytest.mark.parametrize("lattice_name", ["cube", "bcc", "fcc", "a15", "c15"])
 @pytest.mark.parametrize("ncomp", [1, 3, 5])
 def test_fp32_sweeps_match_oracle(oracle, lattice_name, ncomp, xpl):
     """Both FP32 sweep bodies: one x per lane and x pairs per lane (odd nx: a half pair)."""
-    if xpl == 2 and DEGREE[lattice_name] * ncomp > 20:
-        pytest.skip("no pair-lane sweep for this degree x components")
     dims = (5, 4, 3) if lattice_name != "c15" else (3, 2, 2)
     cfg = _config(lattice_name, ncomp, gpe.FP32, dims)
     (g0, g1), (o0, o1) = _run_both(oracle, cfg, 12, field=FIELD if ncomp > 1 else None, traversal=(8, 1, xpl))
@@ -87,10 +82,10 @@ def test_wide_rows_span_several_waves(oracle, precision, tol, xpl, nx):
 
 
 def test_traversal_setter_validates():
-    cfg = _config("fcc", 5, gpe.FP32, (4, 4, 4))
+    cfg = _config("fcc", 5, gpe.FP64, (4, 4, 4))
     with gpe.Stepper(cfg) as st:
         with pytest.raises(gpe.GpeError, match="EINVAL"):
-            st.set_traversal(8, 1, 2)  # degree 8 x 5 components: no pair-lane sweep
+            st.set_traversal(8, 1, 2)  # FP64 has one x per lane only
         with pytest.raises(gpe.GpeError, match="EINVAL"):
             st.set_traversal(0, 1, 1)
         st.set_traversal(4, 0, 1)
