@@ -150,7 +150,34 @@ FieldRec field_at(const gpe_ctx *c, double t)
             f.bbq[k] = c->bbq[k];
         }
     }
+    f.fbq = (float)f.bq;
+    f.fbqq = (float)f.bqq;
+    for (int k = 0; k < 3; ++k) {
+        f.fbb[k] = (float)f.bb[k];
+        f.fbbq[k] = (float)f.bbq[k];
+    }
+    f.fsigma = (float)f.sigma;
+    f.fksc = (float)(1.0 / (f.sigma * f.sigma));
     return f;
+}
+
+// FP32 mirror of the physics constants (after any change to c->ph).
+void physics_to_float(Physics &ph)
+{
+    PhysicsF &f = ph.f;
+    f.cell = (float)ph.cell;
+    for (int k = 0; k < 3; ++k) {
+        f.p0[k] = (float)ph.p0[k];
+        f.trap2[k] = (float)ph.trap2[k];
+        f.center[k] = (float)ph.center[k];
+    }
+    f.trap_offset = (float)ph.trap_offset;
+    f.c0 = (float)ph.c0;
+    f.c1 = (float)ph.c1;
+    f.c2 = (float)ph.c2;
+    f.alpha = (float)ph.alpha;
+    f.quad_sign = (float)ph.quad_sign;
+    f.q = (float)ph.q;
 }
 
 bool quad_active(const gpe_ctx *c)
@@ -330,6 +357,7 @@ int gpe_create(const gpe_config *cfg, gpe_ctx **out)
     ph.alpha = cfg->alpha;
     ph.quad_sign = cfg->quad_sign;
     ph.q = cfg->q;
+    physics_to_float(ph);
 
     int rc = GPE_OK;
     hipError_t e;
@@ -749,6 +777,7 @@ int gpe_set_trap(gpe_ctx *c, const double trap[3])
         c->ph.trap[k] = trap[k];
         c->ph.trap2[k] = trap[k] * trap[k];
     }
+    physics_to_float(c->ph);
     return GPE_OK;
 }
 
@@ -1085,7 +1114,8 @@ int gpe_ensemble_create(gpe_ctx *const *members, int n, gpe_ensemble **out)
     int begin = 0;
     for (int i = 0; i < n; ++i) {
         const gpe_ctx *c = e->m[i];
-        if (i == 0 || c->gs.deg != e->groups.back().deg || c->gs.xpl != e->groups.back().xpl) {
+        if (i == 0 || c->gs.deg != e->groups.back().deg || c->gs.xpl != e->groups.back().xpl ||
+            e->groups.back().count == ENS_GROUP_MAX) {
             e->groups.push_back(EnsembleGroup{c->gs.deg, c->gs.xpl, i, 0, 0});
             begin = 0;
         }
@@ -1180,6 +1210,7 @@ int gpe_ensemble_step(gpe_ensemble *e, int nsteps, int graph_sweeps)
         a.parity = k & 1;
         a.sweep = k;
         a.sweep_base = base;
+        for (int i = 0; i < gr.count; ++i) a.begin[i] = e->h[gr.first + i].begin;
         return launch_ensemble_sweep(e->fp64, e->ncomp, gr.deg, gr.xpl, a, gr.blocks, st);
     };
     auto launch_all = [&](int k, const int *base) -> hipError_t {

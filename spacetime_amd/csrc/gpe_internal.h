@@ -71,6 +71,12 @@ struct Tables {
     const int *nface;      // V: faces of unknown v before its zero-weight pad faces
 };
 
+// FP32 copies of the physics constants, converted once on the host for the FP32 sweep.
+struct PhysicsF {
+    float cell, p0[3], trap2[3], trap_offset, center[3];
+    float c0, c1, c2, alpha, quad_sign, q;
+};
+
 struct Physics {
     double cell;           // s * block_width
     double p0[3];
@@ -79,6 +85,7 @@ struct Physics {
     double trap_offset, trap_off_time;
     double c0, c1, c2, alpha, quad_sign, q;
     double center[3];
+    PhysicsF f;            // the same in FP32
 };
 
 // Field and time of one sweep (computed on the host from the ramp).
@@ -87,6 +94,8 @@ struct FieldRec {
     double bqq, bbq[3];
     double t;
     double sigma;         // block-scale factor of the expanding grid (1: static)
+    // FP32 copies for the FP32 sweep: bq, bb[3], bqq, bbq[3], sigma, 1 / sigma^2
+    float fbq, fbb[3], fbqq, fbbq[3], fsigma, fksc, pad[2];
 };
 
 enum Mode { MODE_UPDATE = 0, MODE_EULER = 1, MODE_ITP = 2, MODE_HPSI = 3 };
@@ -167,12 +176,17 @@ struct EnsembleMember {
     int blocks;               // workgroups of this member
 };
 
+// Members per launch: their first workgroups travel in the kernel arguments, so the
+// workgroup -> member search reads the kernel-argument segment, not the member records.
+constexpr int ENS_GROUP_MAX = 128;
+
 struct EnsembleArgs {
     const EnsembleMember *members;
     int nmember;
     int parity;
     int sweep;                // sweep index within the uploaded field tables
     const int *sweep_base;    // device counter added to sweep (graph replays), may be null
+    int begin[ENS_GROUP_MAX]; // members[i].begin
 };
 
 // All members of one launch share the component count, precision, face degree and lanes.

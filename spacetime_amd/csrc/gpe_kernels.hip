@@ -225,7 +225,8 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
         const int *face = a.tb.face + v * DEG;
         const float *wt = a.tb.w32 + v * DEG;
         const float2 *nbase[DEG];
-        int sh[DEG];
+        int sh[DEG], dxf[DEG];
+        float wf[DEG];
         bool ok0[DEG], ok1[DEG];
 #pragma unroll
         for (int f = 0; f < DEG; ++f) {
@@ -237,11 +238,13 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
                 zn = z;
                 yn = y;
             }
+            dxf[f] = dx;
+            wf[f] = inside ? wt[f] : 0.f;  // wave-uniform: y / z outside the box
             const int s0 = xc + dx;
             const int sc = s0 < 0 ? 0 : (s0 > lxm2 ? lxm2 : s0);
             sh[f] = s0 - sc;  // -1, 0 or +1
-            ok0[f] = inside && (unsigned)(x0 + dx) < (unsigned)g.nx;
-            ok1[f] = inside && (unsigned)(x0 + 1 + dx) < (unsigned)g.nx;
+            ok0[f] = (unsigned)(x0 + dx) < (unsigned)g.nx;
+            ok1[f] = (unsigned)(x0 + 1 + dx) < (unsigned)g.nx;
             nbase[f] = cur + line_index(g, 0, zn, yn, t) + sc;
         }
         const int nf = DEG == 8 ? a.tb.nface[v] : DEG;  // skip zero-weight pad faces
@@ -257,34 +260,48 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
             }
 #pragma unroll
         for (int f = 0; f < DEG; ++f) {
-            const float w = wt[f];
+            const float w = wf[f];
+            if (dxf[f] == 0) {
+                // same x: the pair is (x0, x0 + 1) unshifted; a live value's x is inside the box
+                // (the second value of a lone last x is never stored), so no per-lane masks
 #pragma unroll
-            for (int c = 0; c < C; ++c) {
-                const f4u e = nbv[f][c];
-                // values at x0 + dx and x0 + 1 + dx from the clamped pair
-                float n0r = sh[f] == 0 ? e.x : e.z, n0i = sh[f] == 0 ? e.y : e.w;
-                float n1r = sh[f] == 0 ? e.z : e.x, n1i = sh[f] == 0 ? e.w : e.y;
-                n0r = (ok0[f] && sh[f] >= 0) ? n0r : 0.f;
-                n0i = (ok0[f] && sh[f] >= 0) ? n0i : 0.f;
-                n1r = (ok1[f] && sh[f] <= 0) ? n1r : 0.f;
-                n1i = (ok1[f] && sh[f] <= 0) ? n1i : 0.f;
-                ar[0][c] += w * n0r;
-                ai[0][c] += w * n0i;
-                ar[1][c] += w * n1r;
-                ai[1][c] += w * n1i;
+                for (int c = 0; c < C; ++c) {
+                    const f4u e = nbv[f][c];
+                    ar[0][c] += w * e.x;
+                    ai[0][c] += w * e.y;
+                    ar[1][c] += w * e.z;
+                    ai[1][c] += w * e.w;
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    const f4u e = nbv[f][c];
+                    // values at x0 + dx and x0 + 1 + dx from the clamped pair
+                    float n0r = sh[f] == 0 ? e.x : e.z, n0i = sh[f] == 0 ? e.y : e.w;
+                    float n1r = sh[f] == 0 ? e.z : e.x, n1i = sh[f] == 0 ? e.w : e.y;
+                    n0r = (ok0[f] && sh[f] >= 0) ? n0r : 0.f;
+                    n0i = (ok0[f] && sh[f] >= 0) ? n0i : 0.f;
+                    n1r = (ok1[f] && sh[f] <= 0) ? n1r : 0.f;
+                    n1i = (ok1[f] && sh[f] <= 0) ? n1i : 0.f;
+                    ar[0][c] += w * n0r;
+                    ai[0][c] += w * n0i;
+                    ar[1][c] += w * n1r;
+                    ai[1][c] += w * n1i;
+                }
             }
         }
         const float wsum = a.tb.wsum32[v];
         float sc = 1.f;
         if constexpr (MODE == MODE_ITP) sc = (float)(*a.scale_in);
         const Physics &ph = a.ph;
+        const PhysicsF &pf = ph.f;
         const FieldRec &fr = a.fields[sweep];
-        const float sig = (float)fr.sigma;                            // expanding grid: lengths x sigma
-        const float ksc = sc * (float)(1.0 / (fr.sigma * fr.sigma));  // Laplacian / sigma^2
+        const float sig = fr.fsigma;        // expanding grid: lengths x sigma
+        const float ksc = sc * fr.fksc;     // Laplacian / sigma^2
         const float *pos = a.tb.pos32 + 3 * v;
-        const float cell = (float)ph.cell;
-        const float ry = (float)ph.p0[1] + sig * (cell * (float)y + pos[1]);
-        const float rz = (float)ph.p0[2] + sig * (cell * (float)z + pos[2]);
+        const float cell = pf.cell;
+        const float ry = pf.p0[1] + sig * (cell * (float)y + pos[1]);
+        const float rz = pf.p0[2] + sig * (cell * (float)z + pos[2]);
         const bool trap_on = fr.t < ph.trap_off_time;
         const float coef = (float)a.coef;
         float2 out[2][C];
@@ -298,27 +315,23 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
                 pr[e][c] *= sc;
                 pi[e][c] *= sc;
             }
-            const float rx = (float)ph.p0[0] + sig * (cell * (float)(x0 + e) + pos[0]);
+            const float rx = pf.p0[0] + sig * (cell * (float)(x0 + e) + pos[0]);
             float vtrap = 0.f;
             if (trap_on)
-                vtrap = 0.5f * ((float)ph.trap2[0] * rx * rx + (float)ph.trap2[1] * ry * ry +
-                                (float)ph.trap2[2] * rz * rz) + (float)ph.trap_offset;
+                vtrap = 0.5f * (pf.trap2[0] * rx * rx + pf.trap2[1] * ry * ry + pf.trap2[2] * rz * rz) +
+                        pf.trap_offset;
             float Bx = 0, By = 0, Bz = 0, Qx = 0, Qy = 0, Qz = 0;
             if constexpr (C > 1) {
-                const float dxr = rx - (float)ph.center[0], dyr = ry - (float)ph.center[1],
-                            dzr = rz - (float)ph.center[2];
-                const float bq = (float)fr.bq;
-                Bx = bq * dxr + (float)fr.bb[0];
-                By = bq * dyr + (float)fr.bb[1];
-                Bz = -2.f * bq * dzr + (float)fr.bb[2];
-                const float bqq = (float)fr.bqq;
-                Qx = bqq * dxr + (float)fr.bbq[0];
-                Qy = bqq * dyr + (float)fr.bbq[1];
-                Qz = -2.f * bqq * dzr + (float)fr.bbq[2];
+                const float dxr = rx - pf.center[0], dyr = ry - pf.center[1], dzr = rz - pf.center[2];
+                Bx = fr.fbq * dxr + fr.fbb[0];
+                By = fr.fbq * dyr + fr.fbb[1];
+                Bz = -2.f * fr.fbq * dzr + fr.fbb[2];
+                Qx = fr.fbqq * dxr + fr.fbbq[0];
+                Qy = fr.fbqq * dyr + fr.fbbq[1];
+                Qz = -2.f * fr.fbqq * dzr + fr.fbbq[2];
             }
-            add_local_h<float, C>(pr[e], pi[e], vtrap, Bx, By, Bz, Qx, Qy, Qz, (float)ph.c0, (float)ph.c1,
-                                  (float)ph.c2, (float)ph.alpha, (float)ph.quad_sign, (float)ph.q, a.quad, a.loss,
-                                  hr, hi);
+            add_local_h<float, C>(pr[e], pi[e], vtrap, Bx, By, Bz, Qx, Qy, Qz, pf.c0, pf.c1, pf.c2, pf.alpha,
+                                  pf.quad_sign, pf.q, a.quad, a.loss, hr, hi);
             const bool live = e == 0 ? act0 : act1;
 #pragma unroll
             for (int c = 0; c < C; ++c) {
@@ -396,13 +409,13 @@ __global__ __launch_bounds__(256) void k_ensemble_sweep(EnsembleArgs e)
     int lo = 0, hi = e.nmember - 1;
     while (lo < hi) {  // last member with begin <= b
         const int mid = (lo + hi + 1) >> 1;
-        if (e.members[mid].begin <= b)
+        if (e.begin[mid] <= b)
             lo = mid;
         else
             hi = mid - 1;
     }
     const EnsembleMember &m = e.members[lo];
-    const int local = b - m.begin;
+    const int local = b - e.begin[lo];
     if (local >= m.blocks) return;  // padding workgroup (uniform)
     const SweepArgs &a = m.args[e.parity];
     const int sweep = e.sweep + (e.sweep_base ? *e.sweep_base : 0);
