@@ -19,6 +19,63 @@ namespace gpe {
 
 // ------------------------------------------------------------------------- the sweep --
 
+// Physics and field constants of one sweep in the sweep's precision: the FP32 copies made on
+// the host for FP32 (no per-wave conversions), the FP64 values for FP64.
+template <typename R>
+struct LocalConsts {
+    R cell, p0[3], trap2[3], trap_offset, center[3], c0, c1, c2, alpha, quad_sign, q;
+    R bq, bb[3], bqq, bbq[3], sigma, ksc;
+};
+
+template <typename R>
+__device__ __forceinline__ LocalConsts<R> local_consts(const Physics &ph, const FieldRec &fr)
+{
+    LocalConsts<R> k;
+    if constexpr (std::is_same<R, float>::value) {
+        const PhysicsF &f = ph.f;
+        k.cell = f.cell;
+        for (int i = 0; i < 3; ++i) {
+            k.p0[i] = f.p0[i];
+            k.trap2[i] = f.trap2[i];
+            k.center[i] = f.center[i];
+            k.bb[i] = fr.fbb[i];
+            k.bbq[i] = fr.fbbq[i];
+        }
+        k.trap_offset = f.trap_offset;
+        k.c0 = f.c0;
+        k.c1 = f.c1;
+        k.c2 = f.c2;
+        k.alpha = f.alpha;
+        k.quad_sign = f.quad_sign;
+        k.q = f.q;
+        k.bq = fr.fbq;
+        k.bqq = fr.fbqq;
+        k.sigma = fr.fsigma;
+        k.ksc = fr.fksc;
+    } else {
+        k.cell = ph.cell;
+        for (int i = 0; i < 3; ++i) {
+            k.p0[i] = ph.p0[i];
+            k.trap2[i] = ph.trap2[i];
+            k.center[i] = ph.center[i];
+            k.bb[i] = fr.bb[i];
+            k.bbq[i] = fr.bbq[i];
+        }
+        k.trap_offset = ph.trap_offset;
+        k.c0 = ph.c0;
+        k.c1 = ph.c1;
+        k.c2 = ph.c2;
+        k.alpha = ph.alpha;
+        k.quad_sign = ph.quad_sign;
+        k.q = ph.q;
+        k.bq = fr.bq;
+        k.bqq = fr.bqq;
+        k.sigma = fr.sigma;
+        k.ksc = 1.0 / (fr.sigma * fr.sigma);
+    }
+    return k;
+}
+
 // One x value per lane (FP64, and FP32 where the neighbour pairs would not fit).
 template <typename R, int C, int DEG, int MODE>
 __device__ __forceinline__ void sweep_body_single(const SweepArgs &a, int sweep, int b, int nb)
@@ -59,7 +116,9 @@ __device__ __forceinline__ void sweep_body_single(const SweepArgs &a, int sweep,
         }
         const int *face = a.tb.face + v * DEG;
         const R *wt = table_w<R>(a.tb) + v * DEG;
-        R wf[DEG];
+        R wu[DEG];    // wave-uniform weight: 0 when the neighbour's y or z is outside the box
+        bool xok[DEG];
+        int dxf[DEG];
         const V2 *nbase[DEG];
         int noff[DEG];
 #pragma unroll
@@ -72,10 +131,11 @@ __device__ __forceinline__ void sweep_body_single(const SweepArgs &a, int sweep,
                 zn = z;
                 yn = y;
             }
-            const int xn = x + dx;
-            const bool xok = (unsigned)xn < (unsigned)g.nx;
+            const int xn = xc + dx;
+            xok[f] = (unsigned)xn < (unsigned)g.nx;
+            dxf[f] = dx;
             noff[f] = xn < 0 ? 0 : (xn >= g.nx ? g.nx - 1 : xn);
-            wf[f] = (inside && xok) ? wt[f] : (R)0;
+            wu[f] = inside ? wt[f] : (R)0;
             nbase[f] = cur + line_index(g, 0, zn, yn, t);
         }
         // degree-8 tables pad some unknowns with zero-weight faces: skip their loads
@@ -92,22 +152,26 @@ __device__ __forceinline__ void sweep_body_single(const SweepArgs &a, int sweep,
                     nb[f][c].y = 0;
                 }
             }
-        // stencil: sum_f w_f psi_t(f); out-of-box neighbours carry weight 0
+        // stencil: sum_f w_f psi_t(f); out-of-box neighbours carry weight 0.  Same-x faces need
+        // no per-lane test (a live lane's own x is inside the box): wave-uniform weight.
 #pragma unroll
-        for (int f = 0; f < DEG; ++f)
+        for (int f = 0; f < DEG; ++f) {
+            const R w = (dxf[f] == 0 || xok[f]) ? wu[f] : (R)0;
 #pragma unroll
             for (int c = 0; c < C; ++c) {
-                ar[c] += wf[f] * nb[f][c].x;
-                ai[c] += wf[f] * nb[f][c].y;
+                ar[c] += w * nb[f][c].x;
+                ai[c] += w * nb[f][c].y;
             }
+        }
         const R wsum = table_wsum<R>(a.tb)[v];
         const Physics &ph = a.ph;
         const FieldRec &fr = a.fields[sweep];
+        const LocalConsts<R> k = local_consts<R>(ph, fr);
         R hr[C], hi[C];
         R sc = (R)1;
         if constexpr (MODE == MODE_ITP) sc = (R)(*a.scale_in);
-        const R sig = (R)fr.sigma;                          // expanding grid: lengths x sigma
-        const R ksc = sc * (R)(1.0 / (fr.sigma * fr.sigma));  // Laplacian / sigma^2
+        const R sig = k.sigma;        // expanding grid: lengths x sigma
+        const R ksc = sc * k.ksc;     // Laplacian / sigma^2
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             hr[c] = ksc * (ar[c] - wsum * pr[c]);
@@ -117,28 +181,24 @@ __device__ __forceinline__ void sweep_body_single(const SweepArgs &a, int sweep,
         }
         // position, trap and field
         const R *pos = table_pos<R>(a.tb) + 3 * v;
-        const R cell = (R)ph.cell;
-        const R rx = (R)ph.p0[0] + sig * (cell * (R)x + pos[0]);
-        const R ry = (R)ph.p0[1] + sig * (cell * (R)y + pos[1]);
-        const R rz = (R)ph.p0[2] + sig * (cell * (R)z + pos[2]);
+        const R rx = k.p0[0] + sig * (k.cell * (R)x + pos[0]);
+        const R ry = k.p0[1] + sig * (k.cell * (R)y + pos[1]);
+        const R rz = k.p0[2] + sig * (k.cell * (R)z + pos[2]);
         R vtrap = 0;
         if (fr.t < ph.trap_off_time)
-            vtrap = (R)0.5 * ((R)ph.trap2[0] * rx * rx + (R)ph.trap2[1] * ry * ry + (R)ph.trap2[2] * rz * rz) +
-                    (R)ph.trap_offset;
+            vtrap = (R)0.5 * (k.trap2[0] * rx * rx + k.trap2[1] * ry * ry + k.trap2[2] * rz * rz) + k.trap_offset;
         R Bx = 0, By = 0, Bz = 0, Qx = 0, Qy = 0, Qz = 0;
         if constexpr (C > 1) {
-            const R dxr = rx - (R)ph.center[0], dyr = ry - (R)ph.center[1], dzr = rz - (R)ph.center[2];
-            const R bq = (R)fr.bq;
-            Bx = bq * dxr + (R)fr.bb[0];
-            By = bq * dyr + (R)fr.bb[1];
-            Bz = (R)-2 * bq * dzr + (R)fr.bb[2];
-            const R bqq = (R)fr.bqq;
-            Qx = bqq * dxr + (R)fr.bbq[0];
-            Qy = bqq * dyr + (R)fr.bbq[1];
-            Qz = (R)-2 * bqq * dzr + (R)fr.bbq[2];
+            const R dxr = rx - k.center[0], dyr = ry - k.center[1], dzr = rz - k.center[2];
+            Bx = k.bq * dxr + k.bb[0];
+            By = k.bq * dyr + k.bb[1];
+            Bz = (R)-2 * k.bq * dzr + k.bb[2];
+            Qx = k.bqq * dxr + k.bbq[0];
+            Qy = k.bqq * dyr + k.bbq[1];
+            Qz = (R)-2 * k.bqq * dzr + k.bbq[2];
         }
-        add_local_h<R, C>(pr, pi, vtrap, Bx, By, Bz, Qx, Qy, Qz, (R)ph.c0, (R)ph.c1, (R)ph.c2, (R)ph.alpha,
-                          (R)ph.quad_sign, (R)ph.q, a.quad, a.loss, hr, hi);
+        add_local_h<R, C>(pr, pi, vtrap, Bx, By, Bz, Qx, Qy, Qz, k.c0, k.c1, k.c2, k.alpha, k.quad_sign, k.q,
+                          a.quad, a.loss, hr, hi);
 
         const R coef = (R)a.coef;
 #pragma unroll
