@@ -1,0 +1,158 @@
+"""Independent DEC unit-cell builder: regenerate a lattice's unit-cell table from its atom positions.
+
+The reference's tables (mesh.h, converted to tables/*.json) are the output of its own mesh
+builder (GrossPitaevskii/GPBlocks.cpp:38-301).  This module rebuilds them from first principles,
+without that code, to check the converted tables against the geometry they must describe:
+
+* primal mesh: the Delaunay cells of the periodic atom arrangement (tetrahedra; cospherical
+  groups — the octahedra of FCC, the cubes of the simple-cubic lattice — merged into one cell),
+* unknowns: one per primal 3-cell, at its circumcentre (the dual 0-cell),
+* faces: each polygon shared by two cells; its Hodge ratio is |face| / (|dual edge| * |cell|),
+  the weight of psi_j - psi_i in (L psi)_i for the dual-0-form Laplacian,
+* volumes: the cell volumes.
+
+Coordinates are those of the cubic block [0, a)^3; neighbours in other blocks carry the block
+offset in {-1, 0, 1}^3.  The atom sets below are the textbook ones (simple cubic; BCC; FCC;
+A15 / Cr3Si, Pm-3n; C15 / MgCu2, Fd-3m), up to a translation that `match` finds.
+"""
+import numpy as np
+from scipy.spatial import ConvexHull, Voronoi, cKDTree
+
+ATOMS = {
+    "cube": [(0, 0, 0)],
+    "bcc": [(0, 0, 0), (0.5, 0.5, 0.5)],
+    "fcc": [(0, 0, 0), (0, 0.5, 0.5), (0.5, 0, 0.5), (0.5, 0.5, 0)],
+    "a15": [(0, 0, 0), (0.5, 0.5, 0.5), (0.25, 0, 0.5), (0.75, 0, 0.5), (0.5, 0.25, 0), (0.5, 0.75, 0),
+            (0, 0.5, 0.25), (0, 0.5, 0.75)],
+    "c15": [tuple((np.array(b) + f) % 1.0)
+            for b in [(1 / 8, 1 / 8, 1 / 8), (7 / 8, 3 / 8, 3 / 8),
+                      (0.5, 0.5, 0.5), (0.25, 0.75, 0), (0.75, 0, 0.25), (0, 0.25, 0.75)]
+            for f in [(0, 0, 0), (0, 0.5, 0.5), (0.5, 0, 0.5), (0.5, 0.5, 0)]],
+}
+
+
+def _polygon_area(p):
+    """Area of a planar convex polygon given by its (unordered) vertices (k x 3)."""
+    c = p.mean(0)
+    nrm = np.zeros(3)
+    for i in range(1, len(p) - 1):  # any non-degenerate triangle gives the plane normal
+        nrm = np.cross(p[i] - p[0], p[i + 1] - p[0])
+        if np.linalg.norm(nrm) > 1e-12:
+            break
+    nrm /= np.linalg.norm(nrm)
+    u = p[0] - c
+    u /= np.linalg.norm(u)
+    w = np.cross(nrm, u)
+    ang = np.arctan2((p - c) @ w, (p - c) @ u)
+    q = p[np.argsort(ang)]
+    return 0.5 * abs(sum(np.dot(nrm, np.cross(q[k], q[(k + 1) % len(q)])) for k in range(len(q))))
+
+
+def build(name, a):
+    """Cells of the block [0, a)^3: dict(pos (V,3), vol (V,), faces: list per cell of
+    (dx, dy, dz, target, hodge)).  Works from the Voronoi vertices of the atoms (the
+    circumcentres of the Delaunay cells, cospherical groups already merged): a cell is the
+    convex hull of the atoms nearest its vertex, two cells share a face when they share three
+    or more atoms, and the face is the polygon of the shared atoms."""
+    frac = np.array(ATOMS[name], dtype=np.float64)
+    shifts = np.array([(i, j, k) for i in range(-2, 3) for j in range(-2, 3) for k in range(-2, 3)], float)
+    pts = ((frac[None, :, :] + shifts[:, None, :]).reshape(-1, 3)) * a
+    tol = 1e-7 * a
+    vor = Voronoi(pts)
+    verts = vor.vertices
+    # unique vertices (Qhull may repeat a vertex of a cospherical group)
+    verts = verts[np.all(np.abs(verts) < 4.0 * a, axis=1)]  # drop hull-artefact vertices far out
+    key = np.round(verts / (1e3 * tol)).astype(np.int64)
+    _, first = np.unique(key, axis=0, return_index=True)
+    verts = verts[np.sort(first)]
+    # keep the vertices around the centre block: the central cells and their neighbours
+    near = np.all((verts > -0.6 * a) & (verts < 1.6 * a), axis=1)
+    verts = verts[near]
+    tree = cKDTree(pts)
+    atoms, vols = [], []
+    for v in verts:
+        d, idx = tree.query(v, k=16)
+        sel = idx[d < d[0] + 1e-6 * a]
+        atoms.append(frozenset(int(t) for t in sel))
+        vols.append(ConvexHull(pts[sel]).volume)
+    vols = np.array(vols)
+    inside = np.all((verts >= -tol) & (verts < a - tol), axis=1)
+    central = np.nonzero(inside)[0]
+    central = central[np.lexsort((verts[central, 2], verts[central, 1], verts[central, 0]))]
+    ctree = cKDTree(verts[central])
+
+    def ident(c):
+        p = verts[c]
+        d = np.floor((p + tol) / a)
+        dist, i = ctree.query(p - d * a)
+        return (int(i) if dist < 1e-5 * a else None), d.astype(int)
+
+    vtree = cKDTree(verts)
+    out_faces = []
+    for c in central:
+        row = []
+        for w in vtree.query_ball_point(verts[c], 1.05 * a):
+            if w == c:
+                continue
+            common = atoms[c] & atoms[w]
+            if len(common) < 3:
+                continue
+            poly = pts[sorted(common)]
+            area = _polygon_area(poly)
+            if area < 1e-9 * a * a:  # collinear shared atoms: an edge, not a face
+                continue
+            t, d = ident(w)
+            if t is None:
+                raise RuntimeError("neighbour cell outside the identified set")
+            dual = np.linalg.norm(verts[w] - verts[c])
+            row.append((int(d[0]), int(d[1]), int(d[2]), int(t), area / (dual * vols[c])))
+        out_faces.append(row)
+    return dict(pos=verts[central], vol=vols[central], faces=out_faces)
+
+
+def compare(name, table):
+    """Rebuild `name` at the table's block width and compare: returns dict(topology (bool),
+    max_hodge_diff, max_volume_diff) with table a spacetime_amd.lattice.Lattice."""
+    a = table.block_width
+    b = build(name, a)
+    m = match(b, table.pos, a)
+    if m is None:
+        return dict(topology=False, max_hodge_diff=np.inf, max_volume_diff=np.inf)
+    t, perm = m
+    inv = {int(p): i for i, p in enumerate(perm)}
+    blk = np.floor((b["pos"][perm] + t) / a + 1e-9)  # block of each rebuilt cell after the translation
+    topo, hmax = True, 0.0
+    for i in range(table.V):
+        fs, fe = table.face_start[i], table.face_start[i + 1]
+        want = sorted((tuple(int(v) for v in table.face_d[f]), int(table.face_tgt[f]), float(table.face_hodge[f]))
+                      for f in range(fs, fe))
+        got = []
+        for dx, dy, dz, tgt, h in b["faces"][perm[i]]:
+            j = inv[tgt]
+            off = np.array([dx, dy, dz]) + blk[j] - blk[i]
+            got.append((tuple(int(v) for v in off), j, h))
+        got.sort()
+        if [w[:2] for w in want] != [g[:2] for g in got]:
+            topo = False
+            break
+        hmax = max(hmax, max(abs(w[2] - g[2]) for w, g in zip(want, got)))
+    return dict(topology=topo, max_hodge_diff=hmax, max_volume_diff=float(np.abs(b["vol"][perm] - table.vol).max()))
+
+
+def match(built, table_pos, a, tol=1e-6):
+    """Translation t and permutation perm with built['pos'][perm[i]] + t == table_pos[i] (mod a),
+    or None."""
+    P = np.asarray(table_pos, float)
+    B = built["pos"]
+    for j in range(len(B)):
+        t = P[0] - B[j]
+        perm = []
+        for p in P:
+            diff = (B + t - p + 0.5 * a) % a - 0.5 * a
+            hit = np.nonzero(np.all(np.abs(diff) < tol * a, axis=1))[0]
+            if len(hit) != 1:
+                break
+            perm.append(int(hit[0]))
+        else:
+            return t, np.array(perm)
+    return None
