@@ -261,13 +261,10 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
         const long long own_line = line_index(g, 0, z, y, v);
         const int lxm2 = (int)g.lx - 2;
 
-        // per-lane 32-bit byte offsets from wave-uniform line bases: SGPR base + VGPR offset
-        // addressing instead of a 64-bit per-lane address per access
-        const unsigned own_off = (unsigned)xc * 8u;
         float pr[2][C], pi[2][C], qr[2][C], qi[2][C], ar[2][C], ai[2][C];
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-            const f4a o = *(const f4a *)((const char *)(cur + own_line + c * cstride) + own_off);
+            const f4a o = *(const f4a *)(cur + own_line + c * cstride + xc);
             pr[0][c] = o.x;
             pi[0][c] = o.y;
             pr[1][c] = o.z;
@@ -278,8 +275,7 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
         if constexpr (MODE == MODE_UPDATE) {
 #pragma unroll
             for (int c = 0; c < C; ++c) {
-                const f4a o = __builtin_nontemporal_load(
-                    (const f4a *)((const char *)(prv + own_line + c * cstride) + own_off));
+                const f4a o = __builtin_nontemporal_load((const f4a *)(prv + own_line + c * cstride + xc));
                 qr[0][c] = o.x;
                 qi[0][c] = o.y;
                 qr[1][c] = o.z;
@@ -289,7 +285,6 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
         const int *face = a.tb.face + v * DEG;
         const float *wt = a.tb.w32 + v * DEG;
         const float2 *nbase[DEG];
-        unsigned noff[DEG];
         int sh[DEG], dxf[DEG];
         float wf[DEG];
         bool ok0[DEG], ok1[DEG];
@@ -310,8 +305,7 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
             sh[f] = s0 - sc;  // -1, 0 or +1
             ok0[f] = (unsigned)(x0 + dx) < (unsigned)g.nx;
             ok1[f] = (unsigned)(x0 + 1 + dx) < (unsigned)g.nx;
-            nbase[f] = cur + line_index(g, 0, zn, yn, t);
-            noff[f] = (unsigned)sc * 8u;
+            nbase[f] = cur + line_index(g, 0, zn, yn, t) + sc;
         }
         const int nf = DEG == 8 ? a.tb.nface[v] : DEG;  // skip zero-weight pad faces
         f4u nbv[DEG][C];
@@ -320,7 +314,7 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 if (f < nf)
-                    nbv[f][c] = *(const f4u *)((const char *)(nbase[f] + c * cstride) + noff[f]);
+                    nbv[f][c] = *(const f4u *)(nbase[f] + c * cstride);
                 else
                     nbv[f][c] = (f4u)(0.f);
             }
@@ -421,7 +415,7 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
         }
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-            float2 *dst = (float2 *)((char *)(nxt + own_line + c * cstride) + own_off);
+            float2 *dst = nxt + own_line + c * cstride + xc;
             if (act1) {
                 f4a o;
                 o.x = out[0][c].x;
