@@ -174,7 +174,7 @@ int gpe_local_fields(gpe_ctx *ctx, int level, int kind, double *out);
 /* Scale a level by a real factor (normalize, Spin1GpeBCC.cu:412-434). */
 int gpe_scale(gpe_ctx *ctx, int level, double factor);
 
-/* Wait for all work of the context. */
+/* Wait for all work of the context (the reference's blocking copies synchronise instead). */
 int gpe_sync(gpe_ctx *ctx);
 
 /* Simulation time of level 0 and the number of sweeps taken since set_state. */
@@ -212,7 +212,8 @@ int gpe_autotune(gpe_ctx *ctx, int trial_sweeps, int *tile_rows, int *xcd_remap,
                  double *ms_per_sweep);
 
 /* Set / read the sweep traversal directly: y-tile rows (1..64), XCD remap (0/1), x values per
- * lane (1, or 2 for FP32).  Results do not depend on it beyond
+ * lane (1, or 2 for FP32).  No counterpart in the reference (fixed thread blocks,
+ * Spin1GpeBCC.cu:39-41).  Results do not depend on it beyond
  * FP32 instruction selection. */
 int gpe_set_traversal(gpe_ctx *ctx, int tile_rows, int xcd_remap, int x_per_lane);
 int gpe_traversal(const gpe_ctx *ctx, int *tile_rows, int *xcd_remap, int *x_per_lane);
@@ -248,8 +249,9 @@ int gpe_vortex_fill(const gpe_config *cfg, const gpe_vortex_params *vp, const do
 /* Fill level 0 on the device: psi_c = sqrt(max(mu - V, 0) / g) (w_c + noise * eps_c), eps_c
  * uniform in the unit disk from a counter-based hash of (seed, unknown, component), so the
  * result does not depend on the launch (w = 1: multiplicative noise; a unit spinor w: spinor
- * noise of relative size `noise`).  weights: ncomp complex values (re, im).  Sets the
- * run time to t like gpe_set_state.  For large grids where host-side states do not pay. */
+ * noise of relative size `noise`).  weights: ncomp complex values (re, im).  Sets the run time
+ * to t like gpe_set_state.  For large grids where host-side states do not pay.  No counterpart
+ * in the reference, whose states come from files or host loops (Spin1GpeBCC.cu:1228-1292). */
 int gpe_fill_thomas_fermi(gpe_ctx *ctx, double g, double mu, const double *weights, double noise,
                           unsigned long long seed, double t);
 
@@ -284,7 +286,9 @@ typedef struct {
 
 int gpe_adjoint(gpe_ctx *ctx, const gpe_adjoint_spec *spec, gpe_adjoint_result *res);
 
-/* Replace the trap frequencies of a context (V = 0.5 sum (trap_a r_a)^2 + offset). */
+/* Replace the trap frequencies of a context (V = 0.5 sum (trap_a r_a)^2 + offset): the
+ * lambda_x,y,z of the reference's trap() (Spin1GpeBCC.cu:112-122), a parameter here so the
+ * adjoint's trap gradients can be checked by finite differences. */
 int gpe_set_trap(gpe_ctx *ctx, const double trap[3]);
 
 #ifdef __cplusplus
