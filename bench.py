@@ -353,6 +353,7 @@ def run_form(cfg, psi0, knots, warmup, steps, log, tune_sweeps=0, traversal=None
         log("%s traversal: %s" % ("fp64" if cfg.precision else "fp32", tune))
         if warmup:
             st.step(warmup)
+        ow = st.observables(0)  # after the start-up transient, outside the timed region
         st.sync()
         _sync()
         t0 = time.perf_counter()
@@ -370,6 +371,7 @@ def run_form(cfg, psi0, knots, warmup, steps, log, tune_sweeps=0, traversal=None
                kernel_node_updates_per_s=nodes / (ms_sweep * 1e-3),
                achieved_gbs=bytes_sweep / (ms_sweep * 1e-3) / 1e9,
                norm_drift=abs(o1.norm - o0.norm) / o0.norm, energy_drift=abs(o1.energy - o0.energy) / abs(o0.energy),
+               energy_drift_timed=abs(o1.energy - ow.energy) / abs(ow.energy),
                energy=o1.energy, norm=o1.norm, traversal=tune)
     log("%s: %d steps in %.3fs, kernel %.4f ms/sweep, %.3e node-updates/s, %.1f GB/s" % (
         "fp64" if cfg.precision else "fp32", steps, wall, ms_sweep, res["node_updates_per_s"], res["achieved_gbs"]))
@@ -507,9 +509,11 @@ def main():
                               log, args.tune_sweeps)
         acc["rel_l2_fp32_vs_fp64"] = float(np.linalg.norm((psi32 - psi64).ravel()) / np.linalg.norm(psi64.ravel()))
         del psi64
-    acc.update(norm_drift_fp32=r32["norm_drift"], energy_drift_fp32=r32["energy_drift"])
+    acc.update(norm_drift_fp32=r32["norm_drift"], energy_drift_fp32=r32["energy_drift"],
+               energy_drift_timed_fp32=r32["energy_drift_timed"])
     if r64:
-        acc.update(norm_drift_fp64=r64["norm_drift"], energy_drift_fp64=r64["energy_drift"])
+        acc.update(norm_drift_fp64=r64["norm_drift"], energy_drift_fp64=r64["energy_drift"],
+                   energy_drift_timed_fp64=r64["energy_drift_timed"])
         acc["speedup_fp32_over_fp64"] = r32["node_updates_per_s"] / r64["node_updates_per_s"]
     del psi32
     cpu, cpu_ref = None, None

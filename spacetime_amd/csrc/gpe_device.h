@@ -61,7 +61,8 @@ __device__ __forceinline__ unsigned fast_div(unsigned n, const FastDiv &f)
     return (t + ((n - t) >> 1)) >> (f.l - 1);
 }
 
-// Wave coordinates of workgroup b of nb (b, nb: the workgroup's place in its own run).
+// Wave coordinates of workgroup b of nb (b, nb: the workgroup's place in its own run) for a
+// traversal with one y row per wave (g.rows == 1).
 // Traversal order, slowest to fastest: y-tile, z, row in tile, unknown v, x-segment.
 __device__ __forceinline__ WaveCoord decode(const Geometry &g, int b, int nb)
 {
@@ -83,6 +84,36 @@ __device__ __forceinline__ WaveCoord decode(const Geometry &g, int b, int nb)
 }
 
 __device__ __forceinline__ WaveCoord decode(const Geometry &g) { return decode(g, blockIdx.x, gridDim.x); }
+
+// Lane coordinates of a traversal with g.rows y rows per wave (x pairs on short lines): the wave
+// covers rows y0 .. y0 + rows - 1 of one (z, v); lane l takes row y0 + (l >> lp_shift) and the
+// pair at x = 2 (xseg * lanes per row + l mod lanes per row).  Order: tile, z, row group, v, x
+// segment, as in decode().  valid is wave-uniform; rows past ny are the caller's to mask.
+struct RowCoord {
+    int x, v, y0, r, z;
+    bool valid;
+};
+
+__device__ __forceinline__ RowCoord decode_rows(const Geometry &g, int b, int nb)
+{
+    RowCoord rc;
+    const unsigned wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const unsigned w = (unsigned)(g.xcd_remap ? logical_block(b, nb) : b) * 4u + wid;
+    const unsigned q0 = fast_div(w, g.f_nseg);
+    const int xseg = (int)(w - q0 * (unsigned)g.nseg);
+    const unsigned q1 = fast_div(q0, g.f_V);
+    rc.v = (int)(q0 - q1 * (unsigned)g.V);
+    const unsigned q2 = fast_div(q1, g.f_ty);
+    const int yg = (int)(q1 - q2 * (unsigned)g.tyr);
+    const unsigned q3 = fast_div(q2, g.f_nz);
+    rc.z = (int)(q2 - q3 * (unsigned)g.nz);
+    rc.y0 = (int)q3 * g.ty + yg * g.rows;
+    rc.valid = ((long long)w < g.waves) && (rc.y0 < g.ny);
+    const int lane = (int)(threadIdx.x & 63);
+    rc.r = lane >> g.lp_shift;
+    rc.x = ((xseg << g.lp_shift) + (lane & ((1 << g.lp_shift) - 1))) * 2;
+    return rc;
+}
 
 __device__ __forceinline__ long long line_index(const Geometry &g, int c, int z, int y, int v)
 {

@@ -247,17 +247,27 @@ FastDiv make_fast_div(unsigned d)
 // Wave indices must fit in 32 bits (the decode's arithmetic).
 bool finish_geometry(Geometry &g)
 {
-    const int per_wave = 64 * g.xpl;
-    g.nseg = (g.nx + per_wave - 1) / per_wave;
     g.ty = g.ty < g.ny ? g.ty : g.ny;
+    // x pairs on a line of at most 64 (32) values fill half (a quarter) of a wave: the wave then
+    // takes 2 (4) consecutive y rows of the tile, a power of two that divides the tile's rows
+    int rows = 1;
+    if (g.xpl == 2) {
+        const int want = g.nx <= 32 ? 4 : (g.nx <= 64 ? 2 : 1);
+        while (rows < want && g.ty % (2 * rows) == 0) rows *= 2;
+    }
+    g.rows = rows;
+    g.lp_shift = rows == 1 ? 6 : (rows == 2 ? 5 : 4);
+    g.tyr = g.ty / rows;
+    const int per_wave = (64 / rows) * g.xpl;
+    g.nseg = (g.nx + per_wave - 1) / per_wave;
     g.ntile = (g.ny + g.ty - 1) / g.ty;
-    g.waves = (long long)g.ntile * g.ty * g.nz * g.V * g.nseg;
+    g.waves = (long long)g.ntile * g.tyr * g.nz * g.V * g.nseg;
     const long long blocks = (g.waves + 3) / 4;
     if (blocks >= (1ll << 30)) return false;
     g.blocks = (int)blocks;
     g.f_nseg = make_fast_div((unsigned)g.nseg);
     g.f_V = make_fast_div((unsigned)g.V);
-    g.f_ty = make_fast_div((unsigned)g.ty);
+    g.f_ty = make_fast_div((unsigned)g.tyr);
     g.f_nz = make_fast_div((unsigned)g.nz);
     return true;
 }
@@ -1055,7 +1065,7 @@ int gpe_adjoint(gpe_ctx *c, const gpe_adjoint_spec *spec, gpe_adjoint_result *re
 
 // Members sharing (face degree, x per lane) form a group launched by one specialised kernel.
 struct EnsembleGroup {
-    int deg, xpl;
+    int deg, form;      // face degree and lane_form of every member
     int first, count;   // members [first, first + count) of the sorted member list
     int blocks;         // workgroups of the group's launch
 };
@@ -1105,7 +1115,8 @@ int gpe_ensemble_create(gpe_ctx *const *members, int n, gpe_ensemble **out)
     if (!e) return GPE_ENOMEM;
     e->m.assign(members, members + n);
     std::stable_sort(e->m.begin(), e->m.end(), [](const gpe_ctx *a, const gpe_ctx *b) {
-        return a->gs.deg != b->gs.deg ? a->gs.deg < b->gs.deg : a->gs.xpl < b->gs.xpl;
+        return a->gs.deg != b->gs.deg ? a->gs.deg < b->gs.deg
+                                      : lane_form(a->gs.xpl, a->gs.rows) < lane_form(b->gs.xpl, b->gs.rows);
     });
     e->ncomp = c0->cfg.ncomp;
     e->fp64 = c0->fp64;
@@ -1114,9 +1125,10 @@ int gpe_ensemble_create(gpe_ctx *const *members, int n, gpe_ensemble **out)
     int begin = 0;
     for (int i = 0; i < n; ++i) {
         const gpe_ctx *c = e->m[i];
-        if (i == 0 || c->gs.deg != e->groups.back().deg || c->gs.xpl != e->groups.back().xpl ||
+        const int form = lane_form(c->gs.xpl, c->gs.rows);
+        if (i == 0 || c->gs.deg != e->groups.back().deg || form != e->groups.back().form ||
             e->groups.back().count == ENS_GROUP_MAX) {
-            e->groups.push_back(EnsembleGroup{c->gs.deg, c->gs.xpl, i, 0, 0});
+            e->groups.push_back(EnsembleGroup{c->gs.deg, form, i, 0, 0});
             begin = 0;
         }
         e->groups.back().count += 1;
@@ -1191,7 +1203,8 @@ int gpe_ensemble_step(gpe_ensemble *e, int nsteps, int graph_sweeps)
         gpe_ctx *c = e->m[i];
         if (!c->level1_valid) return ens_fail(e, GPE_ESTATE, "member not started");
         const Geometry &t = e->trav[i];
-        if (c->gs.blocks != t.blocks || c->gs.xpl != t.xpl || c->gs.ty != t.ty || c->gs.xcd_remap != t.xcd_remap)
+        if (c->gs.blocks != t.blocks || c->gs.xpl != t.xpl || c->gs.rows != t.rows || c->gs.ty != t.ty ||
+            c->gs.xcd_remap != t.xcd_remap)
             return ens_fail(e, GPE_ESTATE, "member traversal changed after the ensemble was created");
         int rc = upload_fields(c, c->sweeps, nsw);
         if (rc) return ens_fail(e, rc, c->err);
@@ -1211,7 +1224,7 @@ int gpe_ensemble_step(gpe_ensemble *e, int nsteps, int graph_sweeps)
         a.sweep = k;
         a.sweep_base = base;
         for (int i = 0; i < gr.count; ++i) a.begin[i] = e->h[gr.first + i].begin;
-        return launch_ensemble_sweep(e->fp64, e->ncomp, gr.deg, gr.xpl, a, gr.blocks, st);
+        return launch_ensemble_sweep(e->fp64, e->ncomp, gr.deg, gr.form, a, gr.blocks, st);
     };
     auto launch_all = [&](int k, const int *base) -> hipError_t {
         for (const EnsembleGroup &gr : e->groups) {

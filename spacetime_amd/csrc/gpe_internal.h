@@ -29,10 +29,13 @@ struct FastDiv {
 struct Geometry {
     int nx, ny, nz, V, deg;
     int C;             // complex components (1, 3, 5)
-    FastDiv f_nseg, f_V, f_ty, f_nz;  // the decode's divisors
+    FastDiv f_nseg, f_V, f_ty, f_nz;  // the decode's divisors (f_ty divides by tyr)
     int xpl;           // consecutive x values per lane (1, or 2 for the FP32 sweep)
-    int nseg;          // ceil(nx / (64 xpl)) wave segments per x-line
+    int rows;          // y rows per wave (1; 2 or 4 for x pairs on lines of <= 64 / <= 32 values)
+    int lp_shift;      // log2 of the lanes per row: 6, 5 or 4
+    int nseg;          // ceil(nx / (64 / rows * xpl)) wave segments per x-line
     int ty;            // y rows per tile of the traversal order
+    int tyr;           // row groups per tile: ty / rows
     int ntile;         // ceil(ny / ty)
     long long lx;      // elements per x-line
     long long waves;   // total waves of a sweep
@@ -48,14 +51,16 @@ __host__ __device__ constexpr bool pair_lanes_ok(int fp64, int deg, int ncomp)
     return !fp64 && deg >= 1 && ncomp >= 1;
 }
 
-// Default: pairs where the DEG * C neighbour pairs fit the register budget without cutting
-// occupancy below 4 waves per SIMD, and they leave no more idle lanes than single values do
-// (x-lines of 48 or 64 would idle half of each wave), or for one component, where the
-// per-wave overhead of single values dominates.
+// Sweep body of a traversal: 1 one x per lane, 2 x pairs, 3 x pairs with 2 or 4 y rows per wave.
+__host__ __device__ constexpr int lane_form(int xpl, int rows) { return xpl == 1 ? 1 : (rows > 1 ? 3 : 2); }
+
+// Default: pairs in FP32.  Measured on spin-1 / spin-2 BCC, FCC and Cube boxes of 96-192 blocks,
+// pairs run 30-40% faster than single values, also at degree 8 with 5 components where they cut
+// occupancy to 2 waves per SIMD; lines of <= 64 values share a wave between 2 or 4 y rows, so
+// pairs idle no more lanes than single values do.
 __host__ __device__ constexpr int default_xpl(int fp64, int deg, int ncomp, int nx)
 {
-    return (pair_lanes_ok(fp64, deg, ncomp) && deg * ncomp <= 20 &&
-            (ncomp == 1 || (nx + 127) / 128 * 128 <= (nx + 63) / 64 * 64)) ? 2 : 1;
+    return (pair_lanes_ok(fp64, deg, ncomp) && nx >= 1) ? 2 : 1;
 }
 
 // Per-context constant tables on the device (weights already scaled by -1/2 / s^2).
@@ -190,7 +195,8 @@ struct EnsembleArgs {
 };
 
 // All members of one launch share the component count, precision, face degree and lanes.
-hipError_t launch_ensemble_sweep(int fp64, int ncomp, int deg, int xpl, const EnsembleArgs &e, int blocks,
+// form: lane_form of the group's traversal
+hipError_t launch_ensemble_sweep(int fp64, int ncomp, int deg, int form, const EnsembleArgs &e, int blocks,
                                  hipStream_t s);
 hipError_t launch_advance_base(int *base, int by, hipStream_t s);
 hipError_t launch_set_base(int *base, int value, hipStream_t s);

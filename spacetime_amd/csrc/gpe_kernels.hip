@@ -239,32 +239,53 @@ __device__ __forceinline__ void sweep_body_single(const SweepArgs &a, int sweep,
 // [0, lx - 2] (inside the line, so inside the allocation) and the two values are shifted
 // back into place, and any value outside the box (x, y or z) is replaced by zero by a
 // select, so the line padding x >= nx is never used whatever it holds.
+// ROWS: the wave holds g.rows consecutive y rows of one (z, v) (lines of <= 64 values); a
+// lane's row is then a per-lane offset from the wave's first row y0, a row past ny is clamped
+// to the last row and masked, and a neighbour row outside the box gets weight 0 per lane.
 typedef float f4a __attribute__((ext_vector_type(4)));
 typedef float f4u __attribute__((ext_vector_type(4), aligned(8)));
 typedef float f2n __attribute__((ext_vector_type(2)));
 
-template <int C, int DEG, int MODE>
+template <int C, int DEG, int MODE, bool ROWS>
 __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, int b, int nb)
 {
     const Geometry &g = a.g;
-    const WaveCoord wc = decode(g, b, nb);
+    int x0, v, y0, z, r = 0;
+    bool valid;
+    if constexpr (ROWS) {
+        const RowCoord rc = decode_rows(g, b, nb);
+        x0 = rc.x, v = rc.v, y0 = rc.y0, z = rc.z, r = rc.r, valid = rc.valid;
+    } else {
+        const WaveCoord wc = decode(g, b, nb);
+        x0 = wc.x, v = wc.v, y0 = wc.y, z = wc.z, valid = wc.valid;
+    }
     double part = 0.0;
 
-    if (wc.valid) {
-        const int x0 = wc.x, v = wc.v, y = wc.y, z = wc.z;
-        const bool act0 = x0 < g.nx, act1 = x0 + 1 < g.nx;
-        const int xc = act0 ? x0 : ((g.nx - 1) & ~1);  // even, so xc + 1 <= lx - 1
+    if (valid) {
+        // the lane's row: y0 (+ r under ROWS, clamped into the box) at element offset roff
+        bool rowok = true;
+        int y = y0, roff = 0;
+        const int rowstride = g.V * g.C * (int)g.lx;  // elements between rows y and y + 1
+        if constexpr (ROWS) {
+            rowok = y0 + r < g.ny;
+            const int rr = rowok ? r : g.ny - 1 - y0;
+            y = y0 + rr;
+            roff = rr * rowstride;
+        }
+        const bool act0 = rowok && x0 < g.nx, act1 = rowok && x0 + 1 < g.nx;
+        const int xc = x0 < g.nx ? x0 : ((g.nx - 1) & ~1);  // even, so xc + 1 <= lx - 1
+        const int lo = roff + xc;                            // lane offset from the row-y0 line
         const float2 *__restrict__ cur = (const float2 *)a.cur;
         float2 *__restrict__ nxt = (float2 *)a.next;
         const float2 *__restrict__ prv = (const float2 *)(a.prev ? a.prev : a.next);
         const long long cstride = comp_stride(g);
-        const long long own_line = line_index(g, 0, z, y, v);
+        const long long own_line = line_index(g, 0, z, y0, v);
         const int lxm2 = (int)g.lx - 2;
 
         float pr[2][C], pi[2][C], qr[2][C], qi[2][C], ar[2][C], ai[2][C];
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-            const f4a o = *(const f4a *)(cur + own_line + c * cstride + xc);
+            const f4a o = *(const f4a *)(cur + own_line + c * cstride + lo);
             pr[0][c] = o.x;
             pi[0][c] = o.y;
             pr[1][c] = o.z;
@@ -275,7 +296,7 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
         if constexpr (MODE == MODE_UPDATE) {
 #pragma unroll
             for (int c = 0; c < C; ++c) {
-                const f4a o = __builtin_nontemporal_load((const f4a *)(prv + own_line + c * cstride + xc));
+                const f4a o = __builtin_nontemporal_load((const f4a *)(prv + own_line + c * cstride + lo));
                 qr[0][c] = o.x;
                 qi[0][c] = o.y;
                 qr[1][c] = o.z;
@@ -294,18 +315,24 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
             const int dx = (pk & 3) - 1, dy = ((pk >> 2) & 3) - 1, dz = ((pk >> 4) & 3) - 1, t = pk >> 8;
             int zn = z + dz, yn = y + dy;
             const bool inside = (unsigned)zn < (unsigned)g.nz && (unsigned)yn < (unsigned)g.ny;
-            if (!inside) {
-                zn = z;
-                yn = y;
-            }
             dxf[f] = dx;
-            wf[f] = inside ? wt[f] : 0.f;  // wave-uniform: y / z outside the box
+            wf[f] = inside ? wt[f] : 0.f;  // wave-uniform unless ROWS: y / z outside the box
             const int s0 = xc + dx;
             const int sc = s0 < 0 ? 0 : (s0 > lxm2 ? lxm2 : s0);
             sh[f] = s0 - sc;  // -1, 0 or +1
             ok0[f] = (unsigned)(x0 + dx) < (unsigned)g.nx;
             ok1[f] = (unsigned)(x0 + 1 + dx) < (unsigned)g.nx;
-            nbase[f] = cur + line_index(g, 0, zn, yn, t) + sc;
+            if constexpr (ROWS) {
+                // row-y0 line of (zn, t), the lane's own row when the neighbour is outside
+                if ((unsigned)zn >= (unsigned)g.nz) zn = z;
+                nbase[f] = cur + line_index(g, 0, zn, y0, t) + (roff + (inside ? dy * rowstride : 0) + sc);
+            } else {
+                if (!inside) {
+                    zn = z;
+                    yn = y;
+                }
+                nbase[f] = cur + line_index(g, 0, zn, yn, t) + sc;
+            }
         }
         const int nf = DEG == 8 ? a.tb.nface[v] : DEG;  // skip zero-weight pad faces
         f4u nbv[DEG][C];
@@ -415,7 +442,7 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
         }
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-            float2 *dst = nxt + own_line + c * cstride + xc;
+            float2 *dst = nxt + own_line + c * cstride + lo;
             if (act1) {
                 f4a o;
                 o.x = out[0][c].x;
@@ -442,11 +469,14 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
     }
 }
 
+// X: the lane form (lane_form): 1 one x per lane, 2 x pairs, 3 x pairs over 2 or 4 y rows.
 template <typename R, int C, int DEG, int MODE, int X>
 __device__ __forceinline__ void sweep_body(const SweepArgs &a, int sweep, int b, int nb)
 {
-    if constexpr (X == 2)
-        sweep_body_pair<C, DEG, MODE>(a, sweep, b, nb);
+    if constexpr (X == 3)
+        sweep_body_pair<C, DEG, MODE, true>(a, sweep, b, nb);
+    else if constexpr (X == 2)
+        sweep_body_pair<C, DEG, MODE, false>(a, sweep, b, nb);
     else
         sweep_body_single<R, C, DEG, MODE>(a, sweep, b, nb);
 }
@@ -928,9 +958,11 @@ static hipError_t sweep_mode_x(int mode, const SweepArgs &a, hipStream_t s)
 template <typename R, int C, int DEG>
 static hipError_t sweep_mode(int mode, const SweepArgs &a, hipStream_t s)
 {
-    if (a.g.xpl == 1) return sweep_mode_x<R, C, DEG, 1>(mode, a, s);
+    const int form = lane_form(a.g.xpl, a.g.rows);
+    if (form == 1) return sweep_mode_x<R, C, DEG, 1>(mode, a, s);
     if constexpr (pair_lanes_ok(std::is_same<R, double>::value, DEG, C)) {
-        if (a.g.xpl == 2) return sweep_mode_x<R, C, DEG, 2>(mode, a, s);
+        if (form == 2) return sweep_mode_x<R, C, DEG, 2>(mode, a, s);
+        if (form == 3) return sweep_mode_x<R, C, DEG, 3>(mode, a, s);
     }
     return hipErrorInvalidValue;
 }
@@ -992,16 +1024,20 @@ hipError_t launch_observables(int fp64, int ncomp, const ObsArgs &a, hipStream_t
 }
 
 template <typename R, int C, int DEG>
-static hipError_t ens_x(int xpl, const EnsembleArgs &e, int blocks, hipStream_t s)
+static hipError_t ens_x(int form, const EnsembleArgs &e, int blocks, hipStream_t s)
 {
     dim3 grid(blocks), block(256);
-    if (xpl == 1) {
+    if (form == 1) {
         hipLaunchKernelGGL((k_ensemble_sweep<R, C, DEG, 1>), grid, block, 0, s, e);
         return hipGetLastError();
     }
     if constexpr (pair_lanes_ok(std::is_same<R, double>::value, DEG, C)) {
-        if (xpl == 2) {
+        if (form == 2) {
             hipLaunchKernelGGL((k_ensemble_sweep<R, C, DEG, 2>), grid, block, 0, s, e);
+            return hipGetLastError();
+        }
+        if (form == 3) {
+            hipLaunchKernelGGL((k_ensemble_sweep<R, C, DEG, 3>), grid, block, 0, s, e);
             return hipGetLastError();
         }
     }
@@ -1009,31 +1045,32 @@ static hipError_t ens_x(int xpl, const EnsembleArgs &e, int blocks, hipStream_t 
 }
 
 template <typename R, int C>
-static hipError_t ens_deg(int deg, int xpl, const EnsembleArgs &e, int blocks, hipStream_t s)
+static hipError_t ens_deg(int deg, int form, const EnsembleArgs &e, int blocks, hipStream_t s)
 {
     switch (deg) {
-    case 4: return ens_x<R, C, 4>(xpl, e, blocks, s);
-    case 6: return ens_x<R, C, 6>(xpl, e, blocks, s);
-    case 8: return ens_x<R, C, 8>(xpl, e, blocks, s);
+    case 4: return ens_x<R, C, 4>(form, e, blocks, s);
+    case 6: return ens_x<R, C, 6>(form, e, blocks, s);
+    case 8: return ens_x<R, C, 8>(form, e, blocks, s);
     default: return hipErrorInvalidValue;
     }
 }
 
 template <typename R>
-static hipError_t ens_comp(int ncomp, int deg, int xpl, const EnsembleArgs &e, int blocks, hipStream_t s)
+static hipError_t ens_comp(int ncomp, int deg, int form, const EnsembleArgs &e, int blocks, hipStream_t s)
 {
     switch (ncomp) {
-    case 1: return ens_deg<R, 1>(deg, xpl, e, blocks, s);
-    case 3: return ens_deg<R, 3>(deg, xpl, e, blocks, s);
-    case 5: return ens_deg<R, 5>(deg, xpl, e, blocks, s);
+    case 1: return ens_deg<R, 1>(deg, form, e, blocks, s);
+    case 3: return ens_deg<R, 3>(deg, form, e, blocks, s);
+    case 5: return ens_deg<R, 5>(deg, form, e, blocks, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_ensemble_sweep(int fp64, int ncomp, int deg, int xpl, const EnsembleArgs &e, int blocks,
+hipError_t launch_ensemble_sweep(int fp64, int ncomp, int deg, int form, const EnsembleArgs &e, int blocks,
                                  hipStream_t s)
 {
-    return fp64 ? ens_comp<double>(ncomp, deg, xpl, e, blocks, s) : ens_comp<float>(ncomp, deg, xpl, e, blocks, s);
+    return fp64 ? ens_comp<double>(ncomp, deg, form, e, blocks, s)
+                : ens_comp<float>(ncomp, deg, form, e, blocks, s);
 }
 
 hipError_t launch_advance_base(int *base, int by, hipStream_t s)

@@ -81,6 +81,21 @@ def test_wide_rows_span_several_waves(oracle, precision, tol, xpl, nx):
     assert rel_l2(g0, o0) < tol and rel_l2(g1, o1) < tol
 
 
+@pytest.mark.parametrize("lattice_name,ncomp,dims,traversal", [
+    ("bcc", 3, (5, 6, 3), (4, 1, 2)),   # 4 rows per wave; the second tile's rows 6, 7 are past ny
+    ("fcc", 5, (40, 7, 2), (2, 0, 2)),  # 2 rows per wave; row 7 of the last group is past ny
+    ("a15", 1, (33, 5, 2), (4, 1, 2)),  # 2 rows per wave over 33 values (17 lanes of 32)
+    ("cube", 5, (32, 9, 3), (8, 1, 2)),  # 4 rows per wave; three rows of the last group past ny
+    ("c15", 3, (6, 3, 2), (2, 1, 2)),
+])
+def test_packed_rows_match_oracle(oracle, lattice_name, ncomp, dims, traversal):
+    """x pairs on lines of <= 64 values: one wave holds 2 or 4 consecutive y rows."""
+    cfg = _config(lattice_name, ncomp, gpe.FP32, dims)
+    (g0, g1), (o0, o1) = _run_both(oracle, cfg, 10, field=FIELD if ncomp > 1 else None, traversal=traversal)
+    assert rel_l2(g0, o0) < 2e-5
+    assert rel_l2(g1, o1) < 2e-5
+
+
 def test_traversal_setter_validates():
     cfg = _config("fcc", 5, gpe.FP64, (4, 4, 4))
     with gpe.Stepper(cfg) as st:
@@ -139,6 +154,30 @@ def test_itp_matches_oracle(oracle, ncomp):
         obs = st.observables(0)
     assert obs.norm == pytest.approx(1.0, rel=1e-12)
     assert rel_l2(got, ref) < 1e-12
+
+
+@pytest.mark.parametrize("dims,traversal", [((4, 4, 3), None), ((6, 7, 3), (4, 1, 2))])
+def test_fp32_itp_matches_oracle(oracle, dims, traversal):
+    """FP32 imaginary time (FP64 norm and renormalisation): the default traversal and x pairs over
+    4 rows per wave with rows past ny."""
+    from spacetime_amd import lattice
+    cfg = _config("bcc", 3, gpe.FP32, dims)
+    L = oracle.lattice(lattice.get("bcc"))
+    p = oracle_params(oracle, cfg)
+    psi = random_state(cfg.shape, 9)
+    ref = psi.copy()
+    oracle.normalize(L, p, ref)
+    for _ in range(25):
+        oracle.itp_step(L, p, ref, 1e-3)
+    with gpe.Stepper(cfg) as st:
+        if traversal:
+            st.set_traversal(*traversal)
+        st.set_state(psi)
+        st.itp(25, 1e-3)
+        got = st.get_state(0)
+        obs = st.observables(0)
+    assert obs.norm == pytest.approx(1.0, rel=1e-6)
+    assert rel_l2(got, ref) < 5e-5
 
 
 @pytest.mark.parametrize("lattice_name,ncomp,precision", [
