@@ -149,14 +149,26 @@ def run_ensemble(args, log):
     _sync()
     t0 = time.perf_counter()
     bytes_sweep = 0.0
-    for st in steppers:
+    per_member = []
+    for (cfg, _, _), st in zip(members, steppers):
         st.step(args.steps)
         st.sync()
-        bytes_sweep += st.last_step_timing()[1]
+        ms, nb = st.last_step_timing()
+        bytes_sweep += nb
+        per_member.append((cfg.lattice, cfg.dims[0], ms, nb))
     _sync()
     wall_seq = time.perf_counter() - t0
     obs_seq = [st.observables(0) for st in steppers]
     log("sequential: %d steps in %.3fs, %.3e node-updates/s" % (args.steps, wall_seq, nodes * args.steps / wall_seq))
+    by_lattice = {}
+    for lat, n, ms, nb in per_member:
+        a = by_lattice.setdefault(lat, [0.0, 0.0])
+        a[0] += ms
+        a[1] += nb
+    total_ms = sum(a[0] for a in by_lattice.values())
+    for lat, (ms, nb) in sorted(by_lattice.items(), key=lambda kv: -kv[1][0]):
+        log("sequential %-4s: %.3f ms per sweep (%.0f%% of the sum), %.0f GB/s" % (
+            lat, ms, 100 * ms / total_ms, nb / (ms * 1e-3) / 1e9))
 
     prepare()
     G = args.graph_sweeps
@@ -193,13 +205,86 @@ def run_ensemble(args, log):
                    "blocks_per_axis": [32, 96], "unknowns": nodes, "components": 5, "dt": 1e-4,
                    "graph_sweeps": G,
                    "autotuned": [[t["tile_rows"], t["xcd_remap"], t["x_per_lane"]] for t in tune]},
-        "sequential": {"value": nodes * args.steps / wall_seq, "ms_per_step": 1e3 * wall_seq / args.steps},
+        "sequential": {"value": nodes * args.steps / wall_seq, "ms_per_step": 1e3 * wall_seq / args.steps,
+                       "kernel_ms_per_sweep_sum": total_ms,
+                       "members": [[lat, n, round(ms, 5), round(nb / (ms * 1e-3) / 1e9, 1)]
+                                   for lat, n, ms, nb in per_member]},
         "speedup_vs_sequential": wall_seq / wall_ens,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel_ms_per_sweep": ms_sweep,
                      "algorithmic_bytes_per_sweep": bytes_sweep},
         "accuracy": {"max_rel_energy_diff_vs_sequential": rel_e, "max_rel_norm_diff_vs_sequential": rel_n,
                      "max_norm_drift": drift, "finite": finite},
+        "cpu_baseline": None,
+    }
+
+
+def run_lattices(args, log):
+    """FP32 sweep rate of every lattice x component count at ~25M unknowns (the spin-1 BCC 128^3
+    size): cube 293^3, BCC / FCC 128^3, A15 82^3, C15 57^3 blocks; autotuned traversal."""
+    from spacetime_amd import gpe, lattice
+    box = 16.0
+    gmu = 1.0e4
+    mu = 0.5 * (15.0 * gmu / (4.0 * np.pi)) ** 0.4
+    target = 25165824
+    cases = []
+    tot_nodes_steps, tot_s, tot_bytes, tot_kernel_s = 0.0, 0.0, 0.0, 0.0
+    for name in ("cube", "bcc", "fcc", "a15", "c15"):
+        lat = lattice.get(name)
+        n = int(round((target / lat.V) ** (1.0 / 3.0)))
+        for ncomp in (1, 3, 5):
+            kw = dict(c0=gmu)
+            if ncomp == 3:
+                kw.update(c1=500.0, q=0.1)
+            elif ncomp == 5:
+                kw.update(c1=200.0, c2=50.0)
+            cell = box / n
+            probe = gpe.Config.centred_box(name, (n, n, n), cell, ncomp=ncomp, precision=gpe.FP32, dt=1e-4, **kw)
+            # the 3-level leapfrog needs dt * ||H|| < 1
+            dt = min(1e-4, 0.5 / (lat.kinetic_bound(probe.block_scale) + mu))
+            cfg = gpe.Config.centred_box(name, (n, n, n), cell, ncomp=ncomp, precision=gpe.FP32, dt=dt, **kw)
+            weights = np.zeros(ncomp, dtype=np.complex128)
+            weights[ncomp // 2] = 1.0
+            with gpe.Stepper(cfg) as st:
+                if ncomp > 1:
+                    st.set_field(bq=4e-3, bb=(0.0, 0.0, 0.05))
+                st.fill_thomas_fermi(gmu, mu, weights, noise=0.005, seed=17)
+                st.scale(1.0 / np.sqrt(st.observables(0).norm))
+                st.start(gpe.START_EULER)
+                tune = st.autotune(args.tune_sweeps) if args.tune_sweeps else st.traversal()
+                st.step(args.warmup)
+                st.sync()
+                t0 = time.perf_counter()
+                st.step(args.steps)
+                st.sync()
+                wall = time.perf_counter() - t0
+                ms, nb = st.last_step_timing()
+                o = st.observables(0)
+            nodes = float(n ** 3 * lat.V)
+            gbs = nb / (ms * 1e-3) / 1e9
+            cases.append(dict(lattice=name, components=ncomp, blocks=n, unknowns=nodes, dt=dt,
+                              traversal=[tune["tile_rows"], tune["xcd_remap"], tune["x_per_lane"]],
+                              kernel_ms_per_sweep=ms, node_updates_per_s=nodes * args.steps / wall,
+                              achieved_gbs=gbs, frac=gbs / PEAK_HBM_GBS, norm=o.norm, finite=bool(o.finite)))
+            log("%-4s C=%d %3d^3 blocks: %.4f ms/sweep, %.3e node-updates/s, %6.1f GB/s (%.1f%%), traversal %s" % (
+                name, ncomp, n, ms, nodes / (ms * 1e-3), gbs, 100 * gbs / PEAK_HBM_GBS, cases[-1]["traversal"]))
+            tot_nodes_steps += nodes * args.steps
+            tot_s += wall
+            tot_bytes += nb * args.steps
+            tot_kernel_s += ms * 1e-3 * args.steps
+    achieved = tot_bytes / tot_kernel_s / 1e9
+    return {
+        "metric": "FP32 node-updates/s of every lattice x component count at ~25M unknowns",
+        "value": tot_nodes_steps / tot_s,
+        "unit": "node-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * tot_s / (args.steps * len(cases)),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "lattices_25M", "box": box, "lattices": ["cube", "bcc", "fcc", "a15", "c15"],
+                   "components": [1, 3, 5]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None},
+        "cases": cases,
         "cpu_baseline": None,
     }
 
@@ -458,7 +543,7 @@ def main():
     ap.add_argument("--steps", type=int, default=None,
                     help="timed steps (default: 20000 spin1, 10000 scalar, 5000 ensemble)")
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", choices=["spin1", "scalar", "ensemble", "adjoint"], default="spin1")
+    ap.add_argument("--workload", choices=["spin1", "scalar", "ensemble", "adjoint", "lattices"], default="spin1")
     ap.add_argument("--graph-sweeps", type=int, default=20)
     ap.add_argument("--tune-sweeps", type=int, default=10,
                     help="autotuner trial sweeps per traversal candidate (0: default traversal)")
@@ -476,7 +561,8 @@ def main():
         sys.exit(2)
     log = (lambda *a: None) if args.quiet else (lambda m: print("[bench] " + m, file=sys.stderr, flush=True))
     if args.steps is None:
-        args.steps = {"spin1": 20000, "scalar": 10000, "ensemble": 5000, "adjoint": 4000}[args.workload]
+        args.steps = {"spin1": 20000, "scalar": 10000, "ensemble": 5000, "adjoint": 4000,
+                      "lattices": 1000}[args.workload]
     if args.workload == "ensemble" and (args.graph_sweeps <= 0 or args.graph_sweeps % 2 or
                                         args.steps % args.graph_sweeps):
         ap.error("--graph-sweeps must be even and divide --steps")
@@ -486,6 +572,9 @@ def main():
         return
     if args.workload == "adjoint":
         print(json.dumps(run_adjoint(args, log)))
+        return
+    if args.workload == "lattices":
+        print(json.dumps(run_lattices(args, log)))
         return
 
     from spacetime_amd import gpe

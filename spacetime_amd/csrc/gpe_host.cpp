@@ -75,7 +75,7 @@ struct gpe_ctx {
     int device = 0;
     void *level[2] = {nullptr, nullptr};
     // tables
-    int *d_face = nullptr, *d_nface = nullptr;
+    int *d_face = nullptr, *d_nface = nullptr, *d_facex = nullptr;
     float *d_w32 = nullptr, *d_wsum32 = nullptr, *d_pos32 = nullptr;
     double *d_w64 = nullptr, *d_wsum64 = nullptr, *d_pos64 = nullptr, *d_dv = nullptr;
     // fields per sweep
@@ -199,6 +199,7 @@ Tables tables(const gpe_ctx *c)
     t.pos64 = c->d_pos64;
     t.dv = c->d_dv;
     t.nface = c->d_nface;
+    t.facex = c->d_facex;
     return t;
 }
 
@@ -340,12 +341,20 @@ int gpe_create(const gpe_config *cfg, gpe_ctx **out)
     g.C = cfg->ncomp;
     g.xpl = 1;
     g.ty = cfg->ny < 8 ? cfg->ny : 8;
-    const long long align = c->fp64 ? 8 : 16;  // 128-byte line starts
+    // x-lines start on 128-byte boundaries unless that pads a line by more than 1/8 (short lines:
+    // the padding is fetched and written in part with every line), then on 16 / 32 bytes (an
+    // even number of values, which keeps the x-pair accesses 16-byte aligned)
+    const long long align = c->fp64 ? 8 : 16;
     g.lx = ((long long)cfg->nx + align - 1) / align * align;
+    if (8 * g.lx > 9 * (long long)cfg->nx) g.lx = ((long long)cfg->nx + 1) / 2 * 2;
     g.xcd_remap = 1;
     c->gs = g;
     c->gs.xpl = default_xpl(c->fp64, g.deg, cfg->ncomp, cfg->nx);
     if (!finish_geometry(g) || !finish_geometry(c->gs)) {
+        delete c;
+        return GPE_EINVAL;
+    }
+    if ((long long)(g.ny + 2) * g.V * g.C * g.lx >= (1ll << 31)) {  // 32-bit z-plane offsets
         delete c;
         return GPE_EINVAL;
     }
@@ -386,8 +395,15 @@ int gpe_create(const gpe_config *cfg, gpe_ctx **out)
     const double s = cfg->block_scale;
     std::vector<double> w64(V * deg), wsum64(V, 0.0), pos64(3 * V), dv(V);
     std::vector<float> w32(V * deg), wsum32(V), pos32(3 * V);
-    std::vector<int> nface(V);
+    std::vector<int> nface(V), facex(2 * V * deg);
     for (int v = 0; v < V; ++v) {
+        for (int f = 0; f < deg; ++f) {
+            const int pk = c->lat->face[v * deg + f];
+            const long long dy = ((pk >> 2) & 3) - 1, dz = ((pk >> 4) & 3) - 1, t = pk >> 8;
+            facex[2 * (v * deg + f)] = pk;
+            // |offset| < (ny + 2) V C lx, checked against the int range at creation
+            facex[2 * (v * deg + f) + 1] = (int)(((dz * g.ny + dy) * V + (t - v)) * g.C * g.lx);
+        }
         int nf = deg;  // pad faces (zero Hodge weight) trail the real ones
         while (nf > 0 && c->lat->hodge[v * deg + nf - 1] == 0.0) --nf;
         nface[v] = nf;
@@ -416,6 +432,7 @@ int gpe_create(const gpe_config *cfg, gpe_ctx **out)
     };
     up((void **)&c->d_face, c->lat->face, sizeof(int) * V * deg);
     up((void **)&c->d_nface, nface.data(), sizeof(int) * V);
+    up((void **)&c->d_facex, facex.data(), sizeof(int) * 2 * V * deg);
     up((void **)&c->d_w32, w32.data(), sizeof(float) * V * deg);
     up((void **)&c->d_w64, w64.data(), sizeof(double) * V * deg);
     up((void **)&c->d_wsum32, wsum32.data(), sizeof(float) * V);
@@ -445,7 +462,7 @@ int gpe_destroy(gpe_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int l = 0; l < 2; ++l)
         if (c->level[l]) (void)hipFree(c->level[l]);
-    void *ptrs[] = {c->d_face, c->d_nface, c->d_w32, c->d_w64, c->d_wsum32, c->d_wsum64, c->d_pos32,
+    void *ptrs[] = {c->d_face, c->d_nface, c->d_facex, c->d_w32, c->d_w64, c->d_wsum32, c->d_wsum64, c->d_pos32,
                     c->d_pos64, c->d_dv, c->d_fields, c->d_partials, c->d_scalars, c->d_red};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -1079,11 +1096,6 @@ struct gpe_ensemble {
     int *d_base = nullptr;
     int ncomp = 0, fp64 = 0, per = 1;
     hipStream_t stream = nullptr;
-    // graph replays run each group's sweeps as its own chain on one of these streams, forked
-    // from and joined back to `stream`, so independent groups overlap (default priority)
-    std::vector<hipStream_t> aux;
-    std::vector<hipEvent_t> join;
-    hipEvent_t fork = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     int graph_sweeps = 0;
@@ -1144,16 +1156,6 @@ int gpe_ensemble_create(gpe_ctx *const *members, int n, gpe_ensemble **out)
     if (!rc && hipMalloc((void **)&e->d, sizeof(EnsembleMember) * n) != hipSuccess) rc = GPE_ENOMEM;
     if (!rc && hipMalloc((void **)&e->d_base, sizeof(int) * 4) != hipSuccess) rc = GPE_ENOMEM;
     if (!rc && (hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess)) rc = GPE_EDEVICE;
-    const int naux = e->groups.size() > 1 ? std::min<int>((int)e->groups.size(), 4) : 0;
-    if (!rc && naux && hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess) rc = GPE_EDEVICE;
-    for (int i = 0; i < naux && !rc; ++i) {
-        hipStream_t st = nullptr;
-        hipEvent_t ev = nullptr;
-        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) rc = GPE_EDEVICE;
-        else e->aux.push_back(st);
-        if (!rc && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) rc = GPE_EDEVICE;
-        else if (!rc) e->join.push_back(ev);
-    }
     if (rc) {
         gpe_ensemble_destroy(e);
         return rc;
@@ -1172,12 +1174,6 @@ int gpe_ensemble_destroy(gpe_ensemble *e)
     if (e->d_base) (void)hipFree(e->d_base);
     if (e->ev0) (void)hipEventDestroy(e->ev0);
     if (e->ev1) (void)hipEventDestroy(e->ev1);
-    for (hipStream_t st : e->aux) {
-        (void)hipStreamSynchronize(st);
-        (void)hipStreamDestroy(st);
-    }
-    for (hipEvent_t ev : e->join) (void)hipEventDestroy(ev);
-    if (e->fork) (void)hipEventDestroy(e->fork);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return GPE_OK;
@@ -1233,22 +1229,12 @@ int gpe_ensemble_step(gpe_ensemble *e, int nsteps, int graph_sweeps)
         }
         return hipSuccess;
     };
-    // K sweeps of every group, each group a chain on a forked stream when there are several
+    // K sweeps of every group, one group after another: chains of independent groups forked
+    // onto concurrent streams ran 3% slower (their kernels share each XCD's L2)
     auto capture_chains = [&](int K) -> hipError_t {
-        const size_t na = e->aux.size();
         hipError_t err = hipSuccess;
-        if (na) {
-            err = hipEventRecord(e->fork, e->stream);
-            for (size_t i = 0; i < na && err == hipSuccess; ++i) err = hipStreamWaitEvent(e->aux[i], e->fork, 0);
-        }
-        for (size_t g = 0; g < e->groups.size() && err == hipSuccess; ++g) {
-            hipStream_t st = na ? e->aux[g % na] : e->stream;
-            for (int k = 0; k < K && err == hipSuccess; ++k) err = launch_group(e->groups[g], k, e->d_base, st);
-        }
-        for (size_t i = 0; i < na && err == hipSuccess; ++i) {
-            err = hipEventRecord(e->join[i], e->aux[i]);
-            if (err == hipSuccess) err = hipStreamWaitEvent(e->stream, e->join[i], 0);
-        }
+        for (size_t g = 0; g < e->groups.size() && err == hipSuccess; ++g)
+            for (int k = 0; k < K && err == hipSuccess; ++k) err = launch_group(e->groups[g], k, e->d_base, e->stream);
         return err;
     };
     ENSCALL(e, hipEventRecord(e->ev0, e->stream));

@@ -18,7 +18,8 @@ struct FastDiv {
 
 // Device layout of one time level: z, y, unknown v, component, and x fastest,
 //   element(z, y, v, c, x) = ((((z * ny + y) * V + v) * C + c) * lx + x
-// with lx >= nx rounded up so every x-line starts on a 128-byte boundary.  One wave
+// with lx >= nx rounded up so every x-line starts on a 128-byte boundary (or, where that
+// would pad a line by more than 1/8, lx = nx rounded up to even).  One wave
 // handles 64 consecutive x of one (z, y, v) line, so the unknown v, the face table row and
 // the neighbour rows are uniform across the wave and every neighbour load is a contiguous
 // 64-lane access.  Out-of-box neighbours read as zero (Dirichlet, as the reference's
@@ -74,6 +75,8 @@ struct Tables {
     const double *pos64;   // V*3
     const double *dv;      // V: s^3 * dual volume of unknown v
     const int *nface;      // V: faces of unknown v before its zero-weight pad faces
+    const int *facex;      // V*deg pairs (packed face, elements from unknown v's own line to the
+                           // face's neighbour line: ((dz ny + dy) V + t - v) C lx, y / z inside)
 };
 
 // FP32 copies of the physics constants, converted once on the host for the FP32 sweep.

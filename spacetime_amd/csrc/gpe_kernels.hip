@@ -114,7 +114,14 @@ __device__ __forceinline__ void sweep_body_single(const SweepArgs &a, int sweep,
                 qi[c] = o.y;
             }
         }
-        const int *face = a.tb.face + v * DEG;
+        const int2 *fx = (const int2 *)a.tb.facex + v * DEG;  // (packed face, line offset) pairs
+        int face[DEG], fd[DEG];
+#pragma unroll
+        for (int f = 0; f < DEG; ++f) {
+            const int2 e = fx[f];
+            face[f] = e.x;
+            fd[f] = e.y;
+        }
         const R *wt = table_w<R>(a.tb) + v * DEG;
         R wu[DEG];    // wave-uniform weight: 0 when the neighbour's y or z is outside the box
         bool xok[DEG];
@@ -123,20 +130,34 @@ __device__ __forceinline__ void sweep_body_single(const SweepArgs &a, int sweep,
         int noff[DEG];
 #pragma unroll
         for (int f = 0; f < DEG; ++f) {
-            const int pk = face[f];
-            const int dx = (pk & 3) - 1, dy = ((pk >> 2) & 3) - 1, dz = ((pk >> 4) & 3) - 1, t = pk >> 8;
-            int zn = z + dz, yn = y + dy;
-            const bool inside = (unsigned)zn < (unsigned)g.nz && (unsigned)yn < (unsigned)g.ny;
-            if (!inside) {
-                zn = z;
-                yn = y;
-            }
+            const int dx = (face[f] & 3) - 1;
             const int xn = xc + dx;
             xok[f] = (unsigned)xn < (unsigned)g.nx;
             dxf[f] = dx;
             noff[f] = xn < 0 ? 0 : (xn >= g.nx ? g.nx - 1 : xn);
-            wu[f] = inside ? wt[f] : (R)0;
-            nbase[f] = cur + line_index(g, 0, zn, yn, t);
+        }
+        if (z >= 1 && z <= g.nz - 2 && y >= 1 && y <= g.ny - 2) {
+            // every y / z neighbour inside the box: own line plus the table's per-face offsets
+            const V2 *own = cur + own_line;
+#pragma unroll
+            for (int f = 0; f < DEG; ++f) {
+                wu[f] = wt[f];
+                nbase[f] = own + fd[f];
+            }
+        } else {
+#pragma unroll
+            for (int f = 0; f < DEG; ++f) {
+                const int pk = face[f];
+                const int dy = ((pk >> 2) & 3) - 1, dz = ((pk >> 4) & 3) - 1, t = pk >> 8;
+                int zn = z + dz, yn = y + dy;
+                const bool inside = (unsigned)zn < (unsigned)g.nz && (unsigned)yn < (unsigned)g.ny;
+                if (!inside) {
+                    zn = z;
+                    yn = y;
+                }
+                wu[f] = inside ? wt[f] : (R)0;
+                nbase[f] = cur + line_index(g, 0, zn, yn, t);
+            }
         }
         // degree-8 tables pad some unknowns with zero-weight faces: skip their loads
         const int nf = DEG == 8 ? a.tb.nface[v] : DEG;
@@ -303,77 +324,114 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
                 qi[1][c] = o.w;
             }
         }
-        const int *face = a.tb.face + v * DEG;
+        // packed face and own-line-to-neighbour-line offset side by side: one scalar load
+        const int2 *fx = (const int2 *)a.tb.facex + v * DEG;
+        int face[DEG], fd[DEG];
+#pragma unroll
+        for (int f = 0; f < DEG; ++f) {
+            const int2 e = fx[f];
+            face[f] = e.x;
+            fd[f] = e.y;
+        }
         const float *wt = a.tb.w32 + v * DEG;
         const float2 *nbase[DEG];
-        int sh[DEG], dxf[DEG];
+        int sh[DEG], dxf[DEG], scf[DEG];
         float wf[DEG];
         bool ok0[DEG], ok1[DEG];
 #pragma unroll
         for (int f = 0; f < DEG; ++f) {
-            const int pk = face[f];
-            const int dx = (pk & 3) - 1, dy = ((pk >> 2) & 3) - 1, dz = ((pk >> 4) & 3) - 1, t = pk >> 8;
-            int zn = z + dz, yn = y + dy;
-            const bool inside = (unsigned)zn < (unsigned)g.nz && (unsigned)yn < (unsigned)g.ny;
+            const int dx = (face[f] & 3) - 1;
             dxf[f] = dx;
-            wf[f] = inside ? wt[f] : 0.f;  // wave-uniform unless ROWS: y / z outside the box
             const int s0 = xc + dx;
             const int sc = s0 < 0 ? 0 : (s0 > lxm2 ? lxm2 : s0);
+            scf[f] = sc;
             sh[f] = s0 - sc;  // -1, 0 or +1
             ok0[f] = (unsigned)(x0 + dx) < (unsigned)g.nx;
             ok1[f] = (unsigned)(x0 + 1 + dx) < (unsigned)g.nx;
-            if constexpr (ROWS) {
-                // row-y0 line of (zn, t), the lane's own row when the neighbour is outside
-                if ((unsigned)zn >= (unsigned)g.nz) zn = z;
-                nbase[f] = cur + line_index(g, 0, zn, y0, t) + (roff + (inside ? dy * rowstride : 0) + sc);
-            } else {
-                if (!inside) {
-                    zn = z;
-                    yn = y;
+        }
+        // a wave whose rows and plane are all inside [1, n - 2] has every y / z neighbour inside
+        // the box: its neighbour lines are the own line plus the per-face offsets of the table
+        const int last_row = y0 + (ROWS ? g.rows : 1) - 1;
+        if (z >= 1 && z <= g.nz - 2 && y0 >= 1 && last_row <= g.ny - 2) {
+            const float2 *own = cur + own_line;
+#pragma unroll
+            for (int f = 0; f < DEG; ++f) {
+                wf[f] = wt[f];
+                nbase[f] = own + fd[f] + (roff + scf[f]);
+            }
+        } else {
+#pragma unroll
+            for (int f = 0; f < DEG; ++f) {
+                const int pk = face[f];
+                const int dy = ((pk >> 2) & 3) - 1, dz = ((pk >> 4) & 3) - 1, t = pk >> 8;
+                int zn = z + dz, yn = y + dy;
+                const bool inside = (unsigned)zn < (unsigned)g.nz && (unsigned)yn < (unsigned)g.ny;
+                wf[f] = inside ? wt[f] : 0.f;  // wave-uniform unless ROWS: y / z outside the box
+                if constexpr (ROWS) {
+                    // row-y0 line of (zn, t), the lane's own row when the neighbour is outside
+                    if ((unsigned)zn >= (unsigned)g.nz) zn = z;
+                    nbase[f] = cur + line_index(g, 0, zn, y0, t) + (roff + (inside ? dy * rowstride : 0) + scf[f]);
+                } else {
+                    if (!inside) {
+                        zn = z;
+                        yn = y;
+                    }
+                    nbase[f] = cur + line_index(g, 0, zn, yn, t) + scf[f];
                 }
-                nbase[f] = cur + line_index(g, 0, zn, yn, t) + sc;
             }
         }
-        const int nf = DEG == 8 ? a.tb.nface[v] : DEG;  // skip zero-weight pad faces
-        f4u nbv[DEG][C];
+        // degree-8 tables (FCC) pad the 4-face unknowns with 4 zero-weight faces: skip their
+        // loads.  With 5 components and one row per wave the degree-8 faces go in two batches
+        // of 4, so a wave holds one batch of neighbour pairs at a time (the second batch, loaded
+        // after the first is summed and only by 8-face waves, cannot be hoisted past the
+        // wave-uniform test): 159 instead of 228 VGPRs, 3 waves per SIMD instead of 2, +6-12% on
+        // FCC boxes of 96-128 blocks.  Fewer components or packed rows gain nothing from it.
+        const int nf = DEG == 8 ? a.tb.nface[v] : DEG;
+        constexpr int FB = (DEG == 8 && C == 5 && !ROWS) ? 4 : DEG;
 #pragma unroll
-        for (int f = 0; f < DEG; ++f)
+        for (int f0 = 0; f0 < DEG; f0 += FB) {
+            if (f0 > 0 && f0 >= nf) break;
+            f4u nbv[FB][C];
 #pragma unroll
-            for (int c = 0; c < C; ++c) {
-                if (f < nf)
-                    nbv[f][c] = *(const f4u *)(nbase[f] + c * cstride);
-                else
-                    nbv[f][c] = (f4u)(0.f);
-            }
-#pragma unroll
-        for (int f = 0; f < DEG; ++f) {
-            const float w = wf[f];
-            if (dxf[f] == 0) {
-                // same x: the pair is (x0, x0 + 1) unshifted; a live value's x is inside the box
-                // (the second value of a lone last x is never stored), so no per-lane masks
+            for (int k = 0; k < FB; ++k)
 #pragma unroll
                 for (int c = 0; c < C; ++c) {
-                    const f4u e = nbv[f][c];
-                    ar[0][c] += w * e.x;
-                    ai[0][c] += w * e.y;
-                    ar[1][c] += w * e.z;
-                    ai[1][c] += w * e.w;
+                    if (f0 + k < nf)
+                        nbv[k][c] = *(const f4u *)(nbase[f0 + k] + c * cstride);
+                    else
+                        nbv[k][c] = (f4u)(0.f);
                 }
-            } else {
 #pragma unroll
-                for (int c = 0; c < C; ++c) {
-                    const f4u e = nbv[f][c];
-                    // values at x0 + dx and x0 + 1 + dx from the clamped pair
-                    float n0r = sh[f] == 0 ? e.x : e.z, n0i = sh[f] == 0 ? e.y : e.w;
-                    float n1r = sh[f] == 0 ? e.z : e.x, n1i = sh[f] == 0 ? e.w : e.y;
-                    n0r = (ok0[f] && sh[f] >= 0) ? n0r : 0.f;
-                    n0i = (ok0[f] && sh[f] >= 0) ? n0i : 0.f;
-                    n1r = (ok1[f] && sh[f] <= 0) ? n1r : 0.f;
-                    n1i = (ok1[f] && sh[f] <= 0) ? n1i : 0.f;
-                    ar[0][c] += w * n0r;
-                    ai[0][c] += w * n0i;
-                    ar[1][c] += w * n1r;
-                    ai[1][c] += w * n1i;
+            for (int k = 0; k < FB; ++k) {
+                const int f = f0 + k;
+                const float w = wf[f];
+                if (dxf[f] == 0) {
+                    // same x: the pair is (x0, x0 + 1) unshifted; a live value's x is inside the
+                    // box (the second value of a lone last x is never stored): no per-lane masks
+#pragma unroll
+                    for (int c = 0; c < C; ++c) {
+                        const f4u e = nbv[k][c];
+                        ar[0][c] += w * e.x;
+                        ai[0][c] += w * e.y;
+                        ar[1][c] += w * e.z;
+                        ai[1][c] += w * e.w;
+                    }
+                } else {
+#pragma unroll
+                    for (int c = 0; c < C; ++c) {
+                        const f4u e = nbv[k][c];
+                        // values at x0 + dx and x0 + 1 + dx from the clamped pair
+                        float n0r = sh[f] == 0 ? e.x : e.z, n0i = sh[f] == 0 ? e.y : e.w;
+                        float n1r = sh[f] == 0 ? e.z : e.x, n1i = sh[f] == 0 ? e.w : e.y;
+                        n0r = (ok0[f] && sh[f] >= 0) ? n0r : 0.f;
+                        n0i = (ok0[f] && sh[f] >= 0) ? n0i : 0.f;
+                        n1r = (ok1[f] && sh[f] <= 0) ? n1r : 0.f;
+                        n1i = (ok1[f] && sh[f] <= 0) ? n1i : 0.f;
+                        ar[0][c] += w * n0r;
+                        ai[0][c] += w * n0i;
+                        ar[1][c] += w * n1r;
+                        ai[1][c] += w * n1i;
+                    }
                 }
             }
         }
