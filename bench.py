@@ -44,7 +44,7 @@ PEAK_HBM_GBS = 8000.0
 # (profiles/k_sweep_spin1_bcc_128.md); FETCH_SIZE is scaled by 2.000 and WRITE_SIZE by 1.000,
 # the ratios of bytes moved to counter readings for the 1 GiB streaming copy in the same runs.
 PMC_TRAFFIC = {
-    "spin1_bcc_128^3": 597933.1 * 1024 * 2.000 + 589824.0 * 1024 * 1.000,
+    "spin1_bcc_128^3": 597935.1 * 1024 * 2.000 + 589824.0 * 1024 * 1.000,
 }
 
 
