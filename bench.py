@@ -332,6 +332,30 @@ def run_adjoint(args, log):
     # is not small: a trap change dephases the condensate (phase ~ t omega r^2), so J has large
     # higher derivatives next to its small first derivative near the overlap maximum
     h = 1e-4
+    r32 = out["fp32"]
+    level_bytes = float(np.prod(cfg.dims) * cfg.V * 3 * 8)
+    line = {
+        "metric": "spin-1 BCC 64^3 discrete adjoint: seconds per forward+backward (4000 sweeps, checkpoints every "
+                  "200), gradient error vs central finite differences",
+        "value": r32["wall_s"],
+        "unit": "s per forward+backward",
+        "n_gpus": 1, "steps": nsweeps, "warmup": 1, "ms_per_step": 1e3 * r32["wall_s"] / nsweeps,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "adjoint_spin1_bcc_64^3", "cells": list(cells), "box": box, "sweeps": nsweeps,
+                   "checkpoint_every": K, "parameters": "16 Bz knots + 3 trap frequencies", "objective": "overlap",
+                   "fd_step": h, "fd_scheme": "central at h and h/2, Richardson-combined"},
+        "memory_bytes": {"levels": 2 * level_bytes, "checkpoints": r32["checkpoint_bytes"],
+                         "segment": r32["segment_bytes"]},
+        "forward_ms": r32["forward_ms"], "backward_ms": r32["backward_ms"],
+        "fp64_form": {"value": out["fp64"]["wall_s"], "forward_ms": out["fp64"]["forward_ms"],
+                      "backward_ms": out["fp64"]["backward_ms"]},
+        "accuracy": {"J_fp32": r32["J"], "J_fp64": out["fp64"]["J"],
+                     "grad_fp32": np.concatenate([r32["grad_knots"], r32["grad_trap"]]).tolist(),
+                     "grad_fp64": np.concatenate([out["fp64"]["grad_knots"], out["fp64"]["grad_trap"]]).tolist()},
+        "roofline": None, "cpu_baseline": None,
+    }
+    if args.no_fd:  # timing only (profiling runs): no finite-difference check
+        return line
     fd = RampTrapProblem(cfg64, psi0, nsweeps, K, duration, objective=gpe.OBJ_OVERLAP, target=target, bq=4e-3)
     t0 = time.perf_counter()
 
@@ -362,31 +386,10 @@ def run_adjoint(args, log):
         err_raw[name] = float(np.abs(g - g_h).max() / scale)
     log("max |adjoint - FD| / max |FD| (Richardson FD): fp32 %.2e, fp64 %.2e; against plain h = 1e-4: fp32 %.2e, "
         "fp64 %.2e" % (err["fp32"], err["fp64"], err_raw["fp32"], err_raw["fp64"]))
-    r32 = out["fp32"]
-    level_bytes = float(np.prod(cfg.dims) * cfg.V * 3 * 8)
-    return {
-        "metric": "spin-1 BCC 64^3 discrete adjoint: seconds per forward+backward (4000 sweeps, checkpoints every "
-                  "200), gradient error vs central finite differences",
-        "value": r32["wall_s"],
-        "unit": "s per forward+backward",
-        "n_gpus": 1, "steps": nsweeps, "warmup": 1, "ms_per_step": 1e3 * r32["wall_s"] / nsweeps,
-        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": "adjoint_spin1_bcc_64^3", "cells": list(cells), "box": box, "sweeps": nsweeps,
-                   "checkpoint_every": K, "parameters": "16 Bz knots + 3 trap frequencies", "objective": "overlap",
-                   "fd_step": h, "fd_scheme": "central at h and h/2, Richardson-combined"},
-        "memory_bytes": {"levels": 2 * level_bytes, "checkpoints": r32["checkpoint_bytes"],
-                         "segment": r32["segment_bytes"]},
-        "forward_ms": r32["forward_ms"], "backward_ms": r32["backward_ms"],
-        "fp64_form": {"value": out["fp64"]["wall_s"], "forward_ms": out["fp64"]["forward_ms"],
-                      "backward_ms": out["fp64"]["backward_ms"]},
-        "accuracy": {"max_rel_grad_err_fp32_vs_fd": err["fp32"], "max_rel_grad_err_fp64_vs_fd": err["fp64"],
-                     "max_rel_grad_err_vs_plain_fd_h1e-4": err_raw,
-                     "J_fp32": r32["J"], "J_fp64": out["fp64"]["J"], "grad_fd_richardson": g_fd.tolist(),
-                     "grad_fd_h1e-4": g_h.tolist(),
-                     "grad_fp32": np.concatenate([r32["grad_knots"], r32["grad_trap"]]).tolist(),
-                     "grad_fp64": np.concatenate([out["fp64"]["grad_knots"], out["fp64"]["grad_trap"]]).tolist()},
-        "roofline": None, "cpu_baseline": None,
-    }
+    line["accuracy"].update({"max_rel_grad_err_fp32_vs_fd": err["fp32"], "max_rel_grad_err_fp64_vs_fd": err["fp64"],
+                             "max_rel_grad_err_vs_plain_fd_h1e-4": err_raw, "grad_fd_richardson": g_fd.tolist(),
+                             "grad_fd_h1e-4": g_h.tolist()})
+    return line
 
 
 def prepare_state(cfg, knots, itp_steps, itp_dt, log):
@@ -553,6 +556,7 @@ def main():
     ap.add_argument("--itp-dt", type=float, default=2e-4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-fd", action="store_true", help="adjoint workload: skip the finite-difference check")
     ap.add_argument("--quiet", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
