@@ -85,10 +85,11 @@ __device__ __forceinline__ WaveCoord decode(const Geometry &g, int b, int nb)
 
 __device__ __forceinline__ WaveCoord decode(const Geometry &g) { return decode(g, blockIdx.x, gridDim.x); }
 
-// Lane coordinates of a traversal with g.rows y rows per wave (x pairs on short lines): the wave
-// covers rows y0 .. y0 + rows - 1 of one (z, v); lane l takes row y0 + (l >> lp_shift) and the
-// pair at x = 2 (xseg * lanes per row + l mod lanes per row).  Order: tile, z, row group, v, x
-// segment, as in decode().  valid is wave-uniform; rows past ny are the caller's to mask.
+// Lane coordinates of a traversal with row groups of g.rows y rows (x pairs on lines that one
+// row per wave would leave partly idle): the group's rows * pairs pair slots, row-major, fill
+// g.nseg waves; lane l of segment s takes slot p = 64 s + l, row r = p / pairs of the group
+// and the pair at x = 2 (p - r pairs).  Order: tile, z, row group, v, segment, as in decode().
+// valid is wave-uniform; lanes with r >= rows or a row past ny are the caller's to mask.
 struct RowCoord {
     int x, v, y0, r, z;
     bool valid;
@@ -109,9 +110,10 @@ __device__ __forceinline__ RowCoord decode_rows(const Geometry &g, int b, int nb
     rc.z = (int)(q2 - q3 * (unsigned)g.nz);
     rc.y0 = (int)q3 * g.ty + yg * g.rows;
     rc.valid = ((long long)w < g.waves) && (rc.y0 < g.ny);
-    const int lane = (int)(threadIdx.x & 63);
-    rc.r = lane >> g.lp_shift;
-    rc.x = ((xseg << g.lp_shift) + (lane & ((1 << g.lp_shift) - 1))) * 2;
+    const unsigned p = (unsigned)(xseg * 64) + (threadIdx.x & 63);
+    const unsigned r = fast_div(p, g.f_pairs);
+    rc.r = (int)r;
+    rc.x = 2 * (int)(p - r * (unsigned)g.pairs);
     return rc;
 }
 

@@ -246,21 +246,40 @@ FastDiv make_fast_div(unsigned d)
 
 // Segments, tiles, waves and workgroups of a traversal from nx, ny, nz, V, ty and xpl.
 // Wave indices must fit in 32 bits (the decode's arithmetic).
+// Lanes of a traversal's waves that carry values: rows * pairs pair slots in whole waves.
+static double lane_use(int rows, int pairs)
+{
+    const int slots = rows * pairs;
+    return (double)slots / (64.0 * ((slots + 63) / 64));
+}
+
 bool finish_geometry(Geometry &g)
 {
     g.ty = g.ty < g.ny ? g.ty : g.ny;
-    // x pairs on a line of at most 64 (32) values fill half (a quarter) of a wave: the wave then
-    // takes 2 (4) consecutive y rows of the tile, a power of two that divides the tile's rows
+    g.pairs = (g.nx + 1) / 2;
+    // x pairs: one row per wave leaves lanes idle unless a line holds a multiple of 128 values.
+    // Lines of <= 64 (<= 32) values: the wave's pair slots spread over a group of 2 (4) rows of
+    // the tile (a group size that divides the tile's rows).  Unit cells of <= 12 unknowns with
+    // lines of <= 128 values: the group of 1-8 rows with the best lane use when it fills at
+    // least 5% more lanes (BCC 70^3 +9%, FCC 45^3 +7%).  Larger cells (A15, C15) lose 3-5% with
+    // such groups at 70-90 blocks: their v-lines' shared neighbour lines are then used further
+    // apart in the traversal.
     int rows = 1;
     if (g.xpl == 2) {
         const int want = g.nx <= 32 ? 4 : (g.nx <= 64 ? 2 : 1);
         while (rows < want && g.ty % (2 * rows) == 0) rows *= 2;
+        if (g.V <= 12 && g.nx <= 128) {
+            double best = lane_use(rows, g.pairs) + 0.05;
+            for (int r = 2; r <= 8; r *= 2)
+                if (g.ty % r == 0 && lane_use(r, g.pairs) > best + 1e-9) {
+                    best = lane_use(r, g.pairs);
+                    rows = r;
+                }
+        }
     }
     g.rows = rows;
-    g.lp_shift = rows == 1 ? 6 : (rows == 2 ? 5 : 4);
     g.tyr = g.ty / rows;
-    const int per_wave = (64 / rows) * g.xpl;
-    g.nseg = (g.nx + per_wave - 1) / per_wave;
+    g.nseg = rows > 1 ? (rows * g.pairs + 63) / 64 : (g.nx + 64 * g.xpl - 1) / (64 * g.xpl);
     g.ntile = (g.ny + g.ty - 1) / g.ty;
     g.waves = (long long)g.ntile * g.tyr * g.nz * g.V * g.nseg;
     const long long blocks = (g.waves + 3) / 4;
@@ -270,6 +289,7 @@ bool finish_geometry(Geometry &g)
     g.f_V = make_fast_div((unsigned)g.V);
     g.f_ty = make_fast_div((unsigned)g.tyr);
     g.f_nz = make_fast_div((unsigned)g.nz);
+    g.f_pairs = make_fast_div((unsigned)g.pairs);
     return true;
 }
 
@@ -858,6 +878,12 @@ int gpe_adjoint(gpe_ctx *c, const gpe_adjoint_spec *spec, gpe_adjoint_result *re
     if (!rc && spec->objective == GPE_OBJ_OVERLAP) rc = alloc(&target, lbytes);
     if (!rc) rc = alloc((void **)&d_steps, sizeof(double) * ADJ_COUNT * (size_t)N);
     if (!rc) rc = alloc((void **)&d_part, sizeof(double) * ADJ_COUNT * (size_t)c->g.blocks);
+    // per-sweep sensitivity partials of up to Q consecutive adjoint sweeps, summed in one launch
+    // (one two-stage finalize per sweep cost ~15% of the backward pass in launches)
+    const size_t sweep_part = (size_t)ADJ_COUNT * c->g.blocks;
+    const int Q = (int)std::max<size_t>(1, std::min<size_t>((size_t)K, ((size_t)256 << 20) / (sweep_part * 8)));
+    double *d_qpart = nullptr;
+    if (!rc) rc = alloc((void **)&d_qpart, sizeof(double) * sweep_part * Q);
     if (rc) {
         release();
         return fail(c, rc, "adjoint buffers do not fit");
@@ -991,6 +1017,7 @@ int gpe_adjoint(gpe_ctx *c, const gpe_adjoint_spec *spec, gpe_adjoint_result *re
             a.prev = seg[m - 1 - klo];
             ADJ_CHECK(launch_sweep(c->fp64, c->cfg.ncomp, MODE_UPDATE, a, c->stream), "recompute");
         }
+        int qhi = khi, nq = 0;  // pending sweeps qhi, qhi - 1, ... in slots 0, 1, ...
         for (int k = khi; k >= klo; --k) {
             AdjArgs aa;
             aa.g = c->g;
@@ -1003,11 +1030,15 @@ int gpe_adjoint(gpe_ctx *c, const gpe_adjoint_spec *spec, gpe_adjoint_result *re
             aa.sweep = k;
             aa.quad = quad_active(c);
             aa.coef = 2.0 * sdt;
-            aa.partials = d_part;
+            aa.partials = d_qpart + sweep_part * nq;
             ADJ_CHECK(launch_adjoint(c->fp64, c->cfg.ncomp, aa, c->stream), "adjoint sweep");
-            ADJ_CHECK(launch_finalize(d_part, c->g.blocks, ADJ_COUNT, -1, d_steps + (size_t)ADJ_COUNT * k, c->d_red,
-                                      c->stream),
-                      "adjoint sums");
+            if (++nq == Q || k == klo) {
+                ADJ_CHECK(launch_sum_tables(d_qpart, nq, c->g.blocks, ADJ_COUNT, d_steps + (size_t)ADJ_COUNT * qhi,
+                                            -(long long)ADJ_COUNT, c->stream),
+                          "adjoint sums");
+                qhi = k - 1;
+                nq = 0;
+            }
             std::swap(A, B);
         }
     }

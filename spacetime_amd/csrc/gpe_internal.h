@@ -31,10 +31,12 @@ struct Geometry {
     int nx, ny, nz, V, deg;
     int C;             // complex components (1, 3, 5)
     FastDiv f_nseg, f_V, f_ty, f_nz;  // the decode's divisors (f_ty divides by tyr)
+    FastDiv f_pairs;   // divides a lane's pair slot by `pairs` (rows > 1)
     int xpl;           // consecutive x values per lane (1, or 2 for the FP32 sweep)
-    int rows;          // y rows per wave (1; 2 or 4 for x pairs on lines of <= 64 / <= 32 values)
-    int lp_shift;      // log2 of the lanes per row: 6, 5 or 4
-    int nseg;          // ceil(nx / (64 / rows * xpl)) wave segments per x-line
+    int rows;          // y rows of a row group (1; 2, 4 or 8: x pairs spread over the group's rows)
+    int pairs;         // x pairs per row: ceil(nx / 2)
+    int nseg;          // waves per x-line (rows == 1: ceil(nx / (64 xpl))) or per row group
+                       // (rows > 1: ceil(rows * pairs / 64))
     int ty;            // y rows per tile of the traversal order
     int tyr;           // row groups per tile: ty / rows
     int ntile;         // ceil(ny / ty)
@@ -52,7 +54,7 @@ __host__ __device__ constexpr bool pair_lanes_ok(int fp64, int deg, int ncomp)
     return !fp64 && deg >= 1 && ncomp >= 1;
 }
 
-// Sweep body of a traversal: 1 one x per lane, 2 x pairs, 3 x pairs with 2 or 4 y rows per wave.
+// Sweep body of a traversal: 1 one x per lane, 2 x pairs, 3 x pairs over a group of 2-8 y rows.
 __host__ __device__ constexpr int lane_form(int xpl, int rows) { return xpl == 1 ? 1 : (rows > 1 ? 3 : 2); }
 
 // Default: pairs in FP32.  Measured on spin-1 / spin-2 BCC, FCC and Cube boxes of 96-192 blocks,
@@ -223,6 +225,10 @@ hipError_t launch_observables(int fp64, int ncomp, const ObsArgs &a, hipStream_t
 // scratch: FIN_SCRATCH doubles of device memory owned by the caller's context.
 constexpr int FIN_WIDTH_MAX = 16;
 constexpr int FIN_SCRATCH = 256 * FIN_WIDTH_MAX;
+// column sums of nslot tables of n records of width doubles (table s at partials + s n width),
+// table s's sums to out + s out_stride; one workgroup per table
+hipError_t launch_sum_tables(const double *partials, int nslot, int n, int width, double *out, long long out_stride,
+                             hipStream_t s);
 hipError_t launch_finalize(const double *partials, int n, int width, int maxcol, double *out, double *scratch,
                            hipStream_t s);
 // out = sqrt(target / in[0])

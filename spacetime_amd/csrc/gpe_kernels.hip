@@ -260,9 +260,10 @@ __device__ __forceinline__ void sweep_body_single(const SweepArgs &a, int sweep,
 // [0, lx - 2] (inside the line, so inside the allocation) and the two values are shifted
 // back into place, and any value outside the box (x, y or z) is replaced by zero by a
 // select, so the line padding x >= nx is never used whatever it holds.
-// ROWS: the wave holds g.rows consecutive y rows of one (z, v) (lines of <= 64 values); a
-// lane's row is then a per-lane offset from the wave's first row y0, a row past ny is clamped
-// to the last row and masked, and a neighbour row outside the box gets weight 0 per lane.
+// ROWS: the wave's pairs lie in a group of g.rows consecutive y rows of one (z, v); a lane's
+// row is then a per-lane offset from the group's first row y0, a lane past the group or the box
+// is clamped to the group's last row in the box and masked, and a neighbour row outside the
+// box gets weight 0 per lane.
 typedef float f4a __attribute__((ext_vector_type(4)));
 typedef float f4u __attribute__((ext_vector_type(4), aligned(8)));
 typedef float f2n __attribute__((ext_vector_type(2)));
@@ -288,8 +289,9 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
         int y = y0, roff = 0;
         const int rowstride = g.V * g.C * (int)g.lx;  // elements between rows y and y + 1
         if constexpr (ROWS) {
-            rowok = y0 + r < g.ny;
-            const int rr = rowok ? r : g.ny - 1 - y0;
+            const int nrow = g.ny - y0 < g.rows ? g.ny - y0 : g.rows;  // the group's rows in the box
+            rowok = r < nrow;
+            const int rr = rowok ? r : nrow - 1;
             y = y0 + rr;
             roff = rr * rowstride;
         }
@@ -755,6 +757,29 @@ __global__ __launch_bounds__(256) void k_finalize_stage(const double *partials, 
     }
 }
 
+// Column sums of `nslot` independent partial tables, one workgroup per table: table s holds n
+// records of `width` doubles and its sums go to out + s * out_stride.  A fixed stride order
+// and tree, so identical inputs give identical bits.
+__global__ __launch_bounds__(256) void k_sum_tables(const double *partials, int n, int width, double *out,
+                                                    long long out_stride)
+{
+    __shared__ double red[256];
+    const double *tab = partials + (long long)blockIdx.x * n * width;
+    double *dst = out + (long long)blockIdx.x * out_stride;
+    for (int col = 0; col < width; ++col) {
+        double acc = 0.0;
+        for (int i = (int)threadIdx.x; i < n; i += 256) acc += tab[(long long)i * width + col];
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int st = 128; st > 0; st >>= 1) {
+            if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) dst[col] = red[0];
+        __syncthreads();
+    }
+}
+
 __global__ void k_norm_to_scale(const double *in, double target, double *out)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = sqrt(target / in[0]);
@@ -1158,6 +1183,14 @@ hipError_t launch_finalize(const double *partials, int n, int width, int maxcol,
     hipLaunchKernelGGL(k_finalize_stage, dim3(nb), dim3(256), 0, s, partials, n, width, maxcol, chunk, scratch);
     hipLaunchKernelGGL(k_finalize_stage, dim3(1), dim3(256), 0, s, (const double *)scratch, nb, width, maxcol, nb,
                        out);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_tables(const double *partials, int nslot, int n, int width, double *out, long long out_stride,
+                             hipStream_t s)
+{
+    if (nslot <= 0 || n <= 0 || width <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_sum_tables, dim3(nslot), dim3(256), 0, s, partials, n, width, out, out_stride);
     return hipGetLastError();
 }
 

@@ -87,9 +87,11 @@ def test_wide_rows_span_several_waves(oracle, precision, tol, xpl, nx):
     ("a15", 1, (33, 5, 2), (4, 1, 2)),  # 2 rows per wave over 33 values (17 lanes of 32)
     ("cube", 5, (32, 9, 3), (8, 1, 2)),  # 4 rows per wave; three rows of the last group past ny
     ("c15", 3, (6, 3, 2), (2, 1, 2)),
+    ("bcc", 5, (130, 5, 2), (4, 1, 2)),  # 4-row groups over 5 waves; the second tile's group past ny
+    ("a15", 3, (82, 6, 3), (4, 0, 2)),   # 4-row groups of 41 pairs a row: 3 waves, rows split mid-wave
 ])
 def test_packed_rows_match_oracle(oracle, lattice_name, ncomp, dims, traversal):
-    """x pairs on lines of <= 64 values: one wave holds 2 or 4 consecutive y rows."""
+    """x pairs spread over groups of 2-8 y rows (lines that one row per wave leaves partly idle)."""
     cfg = _config(lattice_name, ncomp, gpe.FP32, dims)
     (g0, g1), (o0, o1) = _run_both(oracle, cfg, 10, field=FIELD if ncomp > 1 else None, traversal=traversal)
     assert rel_l2(g0, o0) < 2e-5
