@@ -221,8 +221,11 @@ def run_ensemble(args, log):
 
 def run_lattices(args, log):
     """FP32 sweep rate of every lattice x component count at ~25M unknowns (the spin-1 BCC 128^3
-    size): cube 293^3, BCC / FCC 128^3, A15 82^3, C15 57^3 blocks; autotuned traversal."""
+    size): cube 293^3, BCC / FCC 128^3, A15 82^3, C15 57^3 blocks; autotuned traversal.  Each case
+    also runs the plain FP64 form from the same start (unless --no-fp64) and reports the relative
+    L2 distance of the final states and both forms' norm and energy drift."""
     from spacetime_amd import gpe, lattice
+    import dataclasses
     box = 16.0
     gmu = 1.0e4
     mu = 0.5 * (15.0 * gmu / (4.0 * np.pi)) ** 0.4
@@ -245,29 +248,53 @@ def run_lattices(args, log):
             cfg = gpe.Config.centred_box(name, (n, n, n), cell, ncomp=ncomp, precision=gpe.FP32, dt=dt, **kw)
             weights = np.zeros(ncomp, dtype=np.complex128)
             weights[ncomp // 2] = 1.0
-            with gpe.Stepper(cfg) as st:
-                if ncomp > 1:
-                    st.set_field(bq=4e-3, bb=(0.0, 0.0, 0.05))
+            field = dict(bq=4e-3, bb=(0.0, 0.0, 0.05)) if ncomp > 1 else None
+            cfg64 = dataclasses.replace(cfg, precision=gpe.FP64)
+            with gpe.Stepper(cfg64) as st:  # the common start: FP64 Thomas-Fermi state with noise
                 st.fill_thomas_fermi(gmu, mu, weights, noise=0.005, seed=17)
                 st.scale(1.0 / np.sqrt(st.observables(0).norm))
-                st.start(gpe.START_EULER)
-                tune = st.autotune(args.tune_sweeps) if args.tune_sweeps else st.traversal()
-                st.step(args.warmup)
-                st.sync()
-                t0 = time.perf_counter()
-                st.step(args.steps)
-                st.sync()
-                wall = time.perf_counter() - t0
-                ms, nb = st.last_step_timing()
-                o = st.observables(0)
+                psi0 = st.get_state(0)
+
+            def run(c, timed):
+                with gpe.Stepper(c) as st:
+                    if field:
+                        st.set_field(**field)
+                    st.set_state(psi0)
+                    o0 = st.observables(0)
+                    st.start(gpe.START_EULER)
+                    tune = st.autotune(args.tune_sweeps) if (timed and args.tune_sweeps) else st.traversal()
+                    st.step(args.warmup)
+                    st.sync()
+                    t0 = time.perf_counter()
+                    st.step(args.steps)
+                    st.sync()
+                    wall = time.perf_counter() - t0
+                    ms, nb = st.last_step_timing()
+                    o1 = st.observables(0)
+                    psi = st.get_state(0)
+                drift = dict(norm_drift=abs(o1.norm - o0.norm) / o0.norm,
+                             energy_drift=abs(o1.energy - o0.energy) / abs(o0.energy))
+                return tune, wall, ms, nb, o1, psi, drift
+
+            tune, wall, ms, nb, o, psi32, drift32 = run(cfg, True)
+            acc = dict(norm_drift_fp32=drift32["norm_drift"], energy_drift_fp32=drift32["energy_drift"])
+            if not args.no_fp64:  # the plain FP64 form from the same start, same steps
+                _, _, _, _, _, psi64, drift64 = run(cfg64, False)
+                acc.update(rel_l2_fp32_vs_fp64=float(np.linalg.norm((psi32 - psi64).ravel()) /
+                                                     np.linalg.norm(psi64.ravel())),
+                           norm_drift_fp64=drift64["norm_drift"], energy_drift_fp64=drift64["energy_drift"])
+                del psi64
+            del psi32, psi0
             nodes = float(n ** 3 * lat.V)
             gbs = nb / (ms * 1e-3) / 1e9
             cases.append(dict(lattice=name, components=ncomp, blocks=n, unknowns=nodes, dt=dt,
                               traversal=[tune["tile_rows"], tune["xcd_remap"], tune["x_per_lane"]],
                               kernel_ms_per_sweep=ms, node_updates_per_s=nodes * args.steps / wall,
-                              achieved_gbs=gbs, frac=gbs / PEAK_HBM_GBS, norm=o.norm, finite=bool(o.finite)))
-            log("%-4s C=%d %3d^3 blocks: %.4f ms/sweep, %.3e node-updates/s, %6.1f GB/s (%.1f%%), traversal %s" % (
-                name, ncomp, n, ms, nodes / (ms * 1e-3), gbs, 100 * gbs / PEAK_HBM_GBS, cases[-1]["traversal"]))
+                              achieved_gbs=gbs, frac=gbs / PEAK_HBM_GBS, norm=o.norm, finite=bool(o.finite),
+                              accuracy=acc))
+            log("%-4s C=%d %3d^3 blocks: %.4f ms/sweep, %.3e node-updates/s, %6.1f GB/s (%.1f%%), traversal %s%s" % (
+                name, ncomp, n, ms, nodes / (ms * 1e-3), gbs, 100 * gbs / PEAK_HBM_GBS, cases[-1]["traversal"],
+                (", FP32 vs FP64 rel L2 %.2e" % acc["rel_l2_fp32_vs_fp64"]) if "rel_l2_fp32_vs_fp64" in acc else ""))
             tot_nodes_steps += nodes * args.steps
             tot_s += wall
             tot_bytes += nb * args.steps
