@@ -38,6 +38,7 @@ extern "C" {
 #define GPE_LATTICE_FCC 2  /* GrossPitaevskiiGpuFcc/mesh.h  12 unknowns / block */
 #define GPE_LATTICE_A15 3  /* GrossPitaevskiiGpuA15/mesh.h  46 unknowns / block */
 #define GPE_LATTICE_C15 4  /* GrossPitaevskiiGpuC15/mesh.h 136 unknowns / block */
+#define GPE_LATTICE_Z 5    /* GrossPitaevskiiGpuZ/mesh.h    80 unknowns / block; block edges (w, sqrt(3) w, w) */
 
 #define GPE_FP32 0 /* FP32 state and stencil arithmetic, FP64 reductions */
 #define GPE_FP64 1 /* FP64 throughout                                    */

@@ -207,10 +207,10 @@ static inline size_t ncells(const orc_params *p) { return (size_t)p->nx * p->ny 
 void orc_position(const orc_lattice *L, const orc_params *p, int x, int y, int z, int v, double r[3])
 {
     /* GPBlocks.cpp:487 */
-    const double s = p->block_scale, w = L->block_width;
-    r[0] = p->p0[0] + s * (x * w + L->pos[3 * v + 0]);
-    r[1] = p->p0[1] + s * (y * w + L->pos[3 * v + 1]);
-    r[2] = p->p0[2] + s * (z * w + L->pos[3 * v + 2]);
+    const double s = p->block_scale, *w = L->block_width;
+    r[0] = p->p0[0] + s * (x * w[0] + L->pos[3 * v + 0]);
+    r[1] = p->p0[1] + s * (y * w[1] + L->pos[3 * v + 1]);
+    r[2] = p->p0[2] + s * (z * w[2] + L->pos[3 * v + 2]);
 }
 
 /* H psi at block (x,y,z) unknown v for all components.
@@ -664,16 +664,16 @@ int orc_gpblocks_integrate(const orc_lattice *L, int kappa, double g, const doub
      * Returns 0, or -1 on allocation failure. */
     orc_params p;
     memset(&p, 0, sizeof(p));
-    const double w = L->block_width;
+    const double *w = L->block_width;
     p.ncomp = 1;
-    p.nx = (int)((unsigned)((maxp[0] - minp[0]) / (block_scale * w)) + 1);
-    p.ny = (int)((unsigned)((maxp[1] - minp[1]) / (block_scale * w)) + 1);
-    p.nz = (int)((unsigned)((maxp[2] - minp[2]) / (block_scale * w)) + 1);
+    p.nx = (int)((unsigned)((maxp[0] - minp[0]) / (block_scale * w[0])) + 1);
+    p.ny = (int)((unsigned)((maxp[1] - minp[1]) / (block_scale * w[1])) + 1);
+    p.nz = (int)((unsigned)((maxp[2] - minp[2]) / (block_scale * w[2])) + 1);
     p.block_scale = block_scale;
     int a;
     for (a = 0; a < 3; ++a) {
         int n = a == 0 ? p.nx : (a == 1 ? p.ny : p.nz);
-        p.p0[a] = 0.5 * (minp[a] + maxp[a] - block_scale * w * n);
+        p.p0[a] = 0.5 * (minp[a] + maxp[a] - block_scale * w[a] * n);
     }
     p.trap[0] = p.trap[1] = p.trap[2] = 1.0;
     p.trap_off_time = 1e300;

@@ -41,7 +41,7 @@ typedef struct {
     const double *face_hodge;  /* [nfaces] Hodge ratio, table units               */
     const double *pos;         /* [V*3] unknown positions in the block, table units */
     const double *vol;         /* [V] dual volume of each unknown, table units    */
-    double block_width;        /* cubic block edge, table units                   */
+    double block_width[3];     /* block edges (x, y, z), table units: cubic but Z  */
 } orc_lattice;
 
 typedef struct {

@@ -87,10 +87,9 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs a)
         const Physics &ph = a.ph;
         const FieldRec &fr = a.fields[a.sweep];
         const R *pos = table_pos<R>(a.tb) + 3 * v;
-        const R cell = (R)ph.cell;
-        const R rx = (R)ph.p0[0] + cell * (R)x + pos[0];
-        const R ry = (R)ph.p0[1] + cell * (R)y + pos[1];
-        const R rz = (R)ph.p0[2] + cell * (R)z + pos[2];
+        const R rx = (R)ph.p0[0] + (R)ph.cell[0] * (R)x + pos[0];
+        const R ry = (R)ph.p0[1] + (R)ph.cell[1] * (R)y + pos[1];
+        const R rz = (R)ph.p0[2] + (R)ph.cell[2] * (R)z + pos[2];
         const bool trap_on = fr.t < ph.trap_off_time;
         R vtrap = 0;
         if (trap_on)
@@ -285,9 +284,9 @@ __global__ __launch_bounds__(256) void k_objective(ObjArgs a)
             n += pr[c] * pr[c] + pi[c] * pi[c];
         }
         const double *pos = a.tb.pos64 + 3 * v;
-        const double rx = a.ph.p0[0] + a.ph.cell * x + pos[0];
-        const double ry = a.ph.p0[1] + a.ph.cell * y + pos[1];
-        const double rz = a.ph.p0[2] + a.ph.cell * z + pos[2];
+        const double rx = a.ph.p0[0] + a.ph.cell[0] * x + pos[0];
+        const double ry = a.ph.p0[1] + a.ph.cell[1] * y + pos[1];
+        const double rz = a.ph.p0[2] + a.ph.cell[2] * z + pos[2];
         p0 = dv * rx * rx * n;
         p1 = dv * ry * ry * n;
         p2 = dv * rz * rz * n;

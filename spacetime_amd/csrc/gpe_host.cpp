@@ -165,8 +165,8 @@ FieldRec field_at(const gpe_ctx *c, double t)
 void physics_to_float(Physics &ph)
 {
     PhysicsF &f = ph.f;
-    f.cell = (float)ph.cell;
     for (int k = 0; k < 3; ++k) {
+        f.cell[k] = (float)ph.cell[k];
         f.p0[k] = (float)ph.p0[k];
         f.trap2[k] = (float)ph.trap2[k];
         f.center[k] = (float)ph.center[k];
@@ -324,7 +324,7 @@ extern "C" {
 
 int gpe_lattice_values(int lattice)
 {
-    if (lattice < 0 || lattice > 4) return GPE_EINVAL;
+    if (lattice < 0 || lattice > 5) return GPE_EINVAL;
     return kLattices[lattice].V;
 }
 
@@ -338,7 +338,7 @@ int gpe_create(const gpe_config *cfg, gpe_ctx **out)
 {
     if (!cfg || !out) return GPE_EINVAL;
     *out = nullptr;
-    if (cfg->lattice < 0 || cfg->lattice > 4) return GPE_EINVAL;
+    if (cfg->lattice < 0 || cfg->lattice > 5) return GPE_EINVAL;
     if (cfg->ncomp != 1 && cfg->ncomp != 3 && cfg->ncomp != 5) return GPE_EINVAL;
     if (cfg->precision != GPE_FP32 && cfg->precision != GPE_FP64) return GPE_EINVAL;
     if (cfg->scheme != GPE_SCHEME_LEAPFROG3 && cfg->scheme != GPE_SCHEME_STAGGERED) return GPE_EINVAL;
@@ -381,7 +381,7 @@ int gpe_create(const gpe_config *cfg, gpe_ctx **out)
     c->level_elems = (long long)cfg->ncomp * g.nz * g.ny * g.V * g.lx;
 
     Physics &ph = c->ph;
-    ph.cell = cfg->block_scale * c->lat->block_width;
+    for (int k = 0; k < 3; ++k) ph.cell[k] = cfg->block_scale * c->lat->block_width[k];
     for (int k = 0; k < 3; ++k) {
         ph.p0[k] = cfg->p0[k];
         ph.trap[k] = cfg->trap[k];

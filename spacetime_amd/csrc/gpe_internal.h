@@ -83,12 +83,12 @@ struct Tables {
 
 // FP32 copies of the physics constants, converted once on the host for the FP32 sweep.
 struct PhysicsF {
-    float cell, p0[3], trap2[3], trap_offset, center[3];
+    float cell[3], p0[3], trap2[3], trap_offset, center[3];
     float c0, c1, c2, alpha, quad_sign, q;
 };
 
 struct Physics {
-    double cell;           // s * block_width
+    double cell[3];        // s * block edges (x, y, z)
     double p0[3];
     double trap[3];        // trap frequencies
     double trap2[3];       // squared trap frequencies

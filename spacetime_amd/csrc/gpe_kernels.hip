@@ -23,7 +23,7 @@ namespace gpe {
 // the host for FP32 (no per-wave conversions), the FP64 values for FP64.
 template <typename R>
 struct LocalConsts {
-    R cell, p0[3], trap2[3], trap_offset, center[3], c0, c1, c2, alpha, quad_sign, q;
+    R cell[3], p0[3], trap2[3], trap_offset, center[3], c0, c1, c2, alpha, quad_sign, q;
     R bq, bb[3], bqq, bbq[3], sigma, ksc;
 };
 
@@ -33,8 +33,8 @@ __device__ __forceinline__ LocalConsts<R> local_consts(const Physics &ph, const 
     LocalConsts<R> k;
     if constexpr (std::is_same<R, float>::value) {
         const PhysicsF &f = ph.f;
-        k.cell = f.cell;
         for (int i = 0; i < 3; ++i) {
+            k.cell[i] = f.cell[i];
             k.p0[i] = f.p0[i];
             k.trap2[i] = f.trap2[i];
             k.center[i] = f.center[i];
@@ -53,8 +53,8 @@ __device__ __forceinline__ LocalConsts<R> local_consts(const Physics &ph, const 
         k.sigma = fr.fsigma;
         k.ksc = fr.fksc;
     } else {
-        k.cell = ph.cell;
         for (int i = 0; i < 3; ++i) {
+            k.cell[i] = ph.cell[i];
             k.p0[i] = ph.p0[i];
             k.trap2[i] = ph.trap2[i];
             k.center[i] = ph.center[i];
@@ -202,9 +202,9 @@ __device__ __forceinline__ void sweep_body_single(const SweepArgs &a, int sweep,
         }
         // position, trap and field
         const R *pos = table_pos<R>(a.tb) + 3 * v;
-        const R rx = k.p0[0] + sig * (k.cell * (R)x + pos[0]);
-        const R ry = k.p0[1] + sig * (k.cell * (R)y + pos[1]);
-        const R rz = k.p0[2] + sig * (k.cell * (R)z + pos[2]);
+        const R rx = k.p0[0] + sig * (k.cell[0] * (R)x + pos[0]);
+        const R ry = k.p0[1] + sig * (k.cell[1] * (R)y + pos[1]);
+        const R rz = k.p0[2] + sig * (k.cell[2] * (R)z + pos[2]);
         R vtrap = 0;
         if (fr.t < ph.trap_off_time)
             vtrap = (R)0.5 * (k.trap2[0] * rx * rx + k.trap2[1] * ry * ry + k.trap2[2] * rz * rz) + k.trap_offset;
@@ -446,9 +446,8 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
         const float sig = fr.fsigma;        // expanding grid: lengths x sigma
         const float ksc = sc * fr.fksc;     // Laplacian / sigma^2
         const float *pos = a.tb.pos32 + 3 * v;
-        const float cell = pf.cell;
-        const float ry = pf.p0[1] + sig * (cell * (float)y + pos[1]);
-        const float rz = pf.p0[2] + sig * (cell * (float)z + pos[2]);
+        const float ry = pf.p0[1] + sig * (pf.cell[1] * (float)y + pos[1]);
+        const float rz = pf.p0[2] + sig * (pf.cell[2] * (float)z + pos[2]);
         const bool trap_on = fr.t < ph.trap_off_time;
         const float coef = (float)a.coef;
         float2 out[2][C];
@@ -462,7 +461,7 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
                 pr[e][c] *= sc;
                 pi[e][c] *= sc;
             }
-            const float rx = pf.p0[0] + sig * (cell * (float)(x0 + e) + pos[0]);
+            const float rx = pf.p0[0] + sig * (pf.cell[0] * (float)(x0 + e) + pos[0]);
             float vtrap = 0.f;
             if (trap_on)
                 vtrap = 0.5f * (pf.trap2[0] * rx * rx + pf.trap2[1] * ry * ry + pf.trap2[2] * rz * rz) +
@@ -631,9 +630,9 @@ __global__ __launch_bounds__(256) void k_observables(ObsArgs a)
         const Physics &ph = a.ph;
         const double *pos = a.tb.pos64 + 3 * v;
         const double sig = a.field.sigma, ik = 1.0 / (sig * sig);
-        const double rx = ph.p0[0] + sig * (ph.cell * x + pos[0]);
-        const double ry = ph.p0[1] + sig * (ph.cell * y + pos[1]);
-        const double rz = ph.p0[2] + sig * (ph.cell * z + pos[2]);
+        const double rx = ph.p0[0] + sig * (ph.cell[0] * x + pos[0]);
+        const double ry = ph.p0[1] + sig * (ph.cell[1] * y + pos[1]);
+        const double rz = ph.p0[2] + sig * (ph.cell[2] * z + pos[2]);
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             kr[c] *= ik;
@@ -838,9 +837,9 @@ __global__ __launch_bounds__(256) void k_fill_tf(Geometry g, Tables tb, Physics 
     const int y = (int)(r % g.ny);
     const int z = (int)(r / g.ny);
     const double *pos = tb.pos64 + 3 * v;
-    const double rx = ph.p0[0] + ph.cell * x + pos[0];
-    const double ry = ph.p0[1] + ph.cell * y + pos[1];
-    const double rz = ph.p0[2] + ph.cell * z + pos[2];
+    const double rx = ph.p0[0] + ph.cell[0] * x + pos[0];
+    const double ry = ph.p0[1] + ph.cell[1] * y + pos[1];
+    const double rz = ph.p0[2] + ph.cell[2] * z + pos[2];
     const double V = 0.5 * (ph.trap2[0] * rx * rx + ph.trap2[1] * ry * ry + ph.trap2[2] * rz * rz) + ph.trap_offset;
     const double a = sqrt(fmax(f.mu - V, 0.0) / f.g);
     const unsigned long long node = (((unsigned long long)z * g.ny + y) * g.nx + x) * g.V + v;

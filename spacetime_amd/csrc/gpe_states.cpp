@@ -146,20 +146,20 @@ int gpe_vortex_fill(const gpe_config *cfg, const gpe_vortex_params *vp, const do
 {
     Profile p;
     if (!cfg || !f || !psi || !profile_of(vp, p)) return GPE_EINVAL;
-    if (cfg->lattice < 0 || cfg->lattice > 4 || cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1) return GPE_EINVAL;
+    if (cfg->lattice < 0 || cfg->lattice > 5 || cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1) return GPE_EINVAL;
     if (cfg->ncomp != 1 && cfg->ncomp != 3 && cfg->ncomp != 5) return GPE_EINVAL;
     const gpe::LatticeTable &lat = gpe::kLattices[cfg->lattice];
     const int V = lat.V;
-    const double s = cfg->block_scale, w = lat.block_width;
+    const double s = cfg->block_scale, *w = lat.block_width;
     const size_t n = (size_t)cfg->nx * cfg->ny * cfg->nz * V;
     for (size_t i = 0; i < 2 * n * (size_t)cfg->ncomp; ++i) psi[i] = 0.0;
     for (int z = 0; z < cfg->nz; ++z)
         for (int y = 0; y < cfg->ny; ++y)
             for (int x = 0; x < cfg->nx; ++x)
                 for (int v = 0; v < V; ++v) {
-                    const double px = cfg->p0[0] + s * (x * w + lat.pos[3 * v + 0]);
-                    const double py = cfg->p0[1] + s * (y * w + lat.pos[3 * v + 1]);
-                    const double pz = cfg->p0[2] + s * (z * w + lat.pos[3 * v + 2]);
+                    const double px = cfg->p0[0] + s * (x * w[0] + lat.pos[3 * v + 0]);
+                    const double py = cfg->p0[1] + s * (y * w[1] + lat.pos[3 * v + 1]);
+                    const double pz = cfg->p0[2] + s * (z * w[2] + lat.pos[3 * v + 2]);
                     const double r = std::sqrt(px * px + py * py);
                     double re = 0.0, im = 0.0;
                     if (!(r < 1e-30)) {

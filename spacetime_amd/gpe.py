@@ -182,10 +182,11 @@ class Config:
 
     @classmethod
     def centred_box(cls, lattice, dims, cell_edge, **kw):
-        """Box of dims blocks of physical edge cell_edge centred on the origin."""
+        """Box of dims blocks centred on the origin; cell_edge is the physical x edge of a block
+        (the blocks are cubes except Z's, whose y edge is sqrt(3) times longer)."""
         lat = _lattice.get(lattice)
         s = cell_edge / lat.block_width
-        p0 = tuple(-0.5 * cell_edge * d for d in dims)
+        p0 = tuple(-0.5 * s * e * d for e, d in zip(lat.block_dims, dims))
         return cls(lattice=lattice, dims=tuple(dims), block_scale=s, p0=p0, **kw)
 
 

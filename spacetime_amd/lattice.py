@@ -1,12 +1,14 @@
-"""DEC unit-cell tables for the Cube, BCC, FCC, A15 and C15 lattices.
+"""DEC unit-cell tables for the Cube, BCC, FCC, A15, C15 and Z lattices.
 
 The tables are the reference's generated ``mesh.h`` constants, converted verbatim to
 JSON by ``tables/convert_mesh_tables.py`` (sources: ``Spin1GpeCube/mesh.h``,
 ``Spin1GpeBCC/mesh.h``, ``GrossPitaevskiiGpuFcc/mesh.h``, ``GrossPitaevskiiGpuA15/mesh.h``,
-``GrossPitaevskiiGpuC15/mesh.h``).
+``GrossPitaevskiiGpuC15/mesh.h``, ``GrossPitaevskiiGpuZ/mesh.h``).
 
-Each lattice has ``V`` unknowns (dual 0-cells) per cubic block of edge ``block_width``
-(table units; physical length = ``block_scale`` * table length).  The Laplacian of the
+Each lattice has ``V`` unknowns (dual 0-cells) per block of edges ``block_dims`` (table units;
+physical length = ``block_scale`` * table length).  The blocks are cubes of edge
+``block_width`` except Z's, whose edges are (w, sqrt(3) w, w) (``block_width`` = w, the x
+edge).  The Laplacian of the
 reference (``Spin1GpeBCC/Spin1GpeBCC.cu:827-865``) is
 
     (L psi)_i = sum_f hodge_f / s^2 * (psi_{t(f)} - psi_i)
@@ -24,7 +26,7 @@ import os
 
 import numpy as np
 
-LATTICES = ("cube", "bcc", "fcc", "a15", "c15")
+LATTICES = ("cube", "bcc", "fcc", "a15", "c15", "z")
 LATTICE_IDS = {name: i for i, name in enumerate(LATTICES)}
 _TABLE_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tables")
 
@@ -40,7 +42,8 @@ class Lattice:
         self.name = name
         self.id = LATTICE_IDS[name]
         self.V = int(t["values_in_block"])
-        self.block_width = float(t["block_width"][0])
+        self.block_dims = np.array(t["block_width"], dtype=np.float64)
+        self.block_width = float(self.block_dims[0])
         self.ref_volume = float(t["volume"])
         self.dual_edge_length = float(t["dual_edge_length"])
         self.provenance = t["provenance"]
@@ -76,7 +79,7 @@ class Lattice:
                     changed = True
         if np.isnan(vol).any():
             raise ValueError("%s table is not connected" % self.name)
-        return vol * (self.block_width ** 3 / vol.sum())
+        return vol * (float(np.prod(self.block_dims)) / vol.sum())
 
     def laplacian_symbol(self, k, block_scale=1.0):
         """Bloch matrix of -1/2 L for wave vector k (physical units).
@@ -91,7 +94,7 @@ class Lattice:
         for f in range(self.nfaces):
             i, j = self.face_src[f], self.face_tgt[f]
             w = -0.5 * self.face_hodge[f] / (s * s)
-            dr = s * (self.face_d[f] * self.block_width + self.pos[j] - self.pos[i])
+            dr = s * (self.face_d[f] * self.block_dims + self.pos[j] - self.pos[i])
             M[i, j] += w * np.exp(1j * np.dot(k, dr))
             M[i, i] -= w
         sq = np.sqrt(self.vol)

@@ -70,16 +70,16 @@ def vortex_fill(cfg, vs):
 
 
 def positions(cfg):
-    """Physical positions, shape (nz, ny, nx, V, 3): p0 + s (i * block_width + pos_v), the
+    """Physical positions, shape (nz, ny, nx, V, 3): p0 + s (i * block edge + pos_v), the
     association order of GPBlocks.cpp:487."""
     lat = _lattice.get(cfg.lattice)
     nx, ny, nz = cfg.dims
     s = cfg.block_scale
-    w = lat.block_width
+    w = lat.block_dims
     out = np.empty((nz, ny, nx, lat.V, 3))
-    out[..., 0] = cfg.p0[0] + s * (np.arange(nx)[None, None, :, None] * w + lat.pos[None, None, None, :, 0])
-    out[..., 1] = cfg.p0[1] + s * (np.arange(ny)[None, :, None, None] * w + lat.pos[None, None, None, :, 1])
-    out[..., 2] = cfg.p0[2] + s * (np.arange(nz)[:, None, None, None] * w + lat.pos[None, None, None, :, 2])
+    out[..., 0] = cfg.p0[0] + s * (np.arange(nx)[None, None, :, None] * w[0] + lat.pos[None, None, None, :, 0])
+    out[..., 1] = cfg.p0[1] + s * (np.arange(ny)[None, :, None, None] * w[1] + lat.pos[None, None, None, :, 1])
+    out[..., 2] = cfg.p0[2] + s * (np.arange(nz)[:, None, None, None] * w[2] + lat.pos[None, None, None, :, 2])
     return out
 
 

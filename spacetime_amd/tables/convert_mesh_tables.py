@@ -1,7 +1,7 @@
 """Convert the reference's per-lattice ``mesh.h`` unit-cell tables into repo-owned JSON.
 
 This is the one-off recipe that produced ``cube.json``, ``bcc.json``, ``fcc.json``,
-``a15.json`` and ``c15.json`` next to it.  It reads the reference checkout (default
+``a15.json``, ``c15.json`` and ``z.json`` next to it.  It reads the reference checkout (default
 ``/root/reference``) as text, so it only runs where that checkout exists; the JSON
 files it writes are the data every other part of the repository uses.
 
@@ -9,7 +9,7 @@ What is extracted from each ``mesh.h`` (the tables are generated constants of th
 reference's weighted-Delaunay builder, ``GrossPitaevskii/GPBlocks.cpp:38-301``):
 
 * ``VALUES_IN_BLOCK`` unknowns (dual 0-cells = circumcentres) per cubic block,
-* ``BLOCK_WIDTH`` (edge of the cubic block in table units), ``VOLUME`` (dual volume
+* ``BLOCK_WIDTH`` (edges of the block in table units; cubic except Z), ``VOLUME`` (dual volume
   of one unknown in table units), ``DUAL_EDGE_LENGTH``,
 * the unknowns' positions inside the block,
 * for every unknown its faces: neighbour block offset (dx, dy, dz) in {-1,0,1}^3,
@@ -23,6 +23,8 @@ Sources and formats:
   nx/ny/nz, target) plus ``indicesAndFaceCounts`` (variable degree 4 or 8)
 * A15   ``GrossPitaevskiiGpuA15/mesh.h:8-432`` int2 offsets, degree 4
 * C15   ``GrossPitaevskiiGpuC15/mesh.h:8-1242`` int2 offsets, degree 4
+* Z     ``GrossPitaevskiiGpuZ/mesh.h:8-738`` int2 offsets, degree 4; the block is not cubic:
+  BLOCK_WIDTH = (w, sqrt(3) w, w) (the hexagonal Z phase in an orthorhombic cell)
 
 Usage: ``python convert_mesh_tables.py [reference_root]``
 """
@@ -40,6 +42,7 @@ SOURCES = {
     "fcc": ("GrossPitaevskiiGpuFcc/mesh.h", "int2"),
     "a15": ("GrossPitaevskiiGpuA15/mesh.h", "int2"),
     "c15": ("GrossPitaevskiiGpuC15/mesh.h", "int2"),
+    "z": ("GrossPitaevskiiGpuZ/mesh.h", "int2"),
 }
 
 NUM = r"[-+]?\d+(?:\.\d*)?(?:[eE][-+]?\d+)?"

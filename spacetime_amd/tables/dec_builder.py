@@ -11,9 +11,12 @@ without that code, to check the converted tables against the geometry they must 
   the weight of psi_j - psi_i in (L psi)_i for the dual-0-form Laplacian,
 * volumes: the cell volumes.
 
-Coordinates are those of the cubic block [0, a)^3; neighbours in other blocks carry the block
-offset in {-1, 0, 1}^3.  The atom sets below are the textbook ones (simple cubic; BCC; FCC;
-A15 / Cr3Si, Pm-3n; C15 / MgCu2, Fd-3m), up to a translation that `match` finds.
+Coordinates are those of the block [0, a) x [0, b) x [0, c) (a cube except for Z); neighbours
+in other blocks carry the block offset in {-1, 0, 1}^3.  The atom sets below are the textbook
+ones (simple cubic; BCC; FCC; A15 / Cr3Si, Pm-3n; C15 / MgCu2, Fd-3m; the Frank-Kasper Z phase,
+Zr4Al3 type, P6/mmm, in its orthorhombic cell of edges (1, sqrt(3), 1): a hexagonal net at
+heights 0 and 1/2 plus the ten sites that Mesh/DelaunayMesh.cpp:511-556 adds),
+up to a translation that `match` finds.
 """
 import numpy as np
 from scipy.spatial import ConvexHull, Voronoi, cKDTree
@@ -28,7 +31,14 @@ ATOMS = {
             for b in [(1 / 8, 1 / 8, 1 / 8), (7 / 8, 3 / 8, 3 / 8),
                       (0.5, 0.5, 0.5), (0.25, 0.75, 0), (0.75, 0, 0.25), (0, 0.25, 0.75)]
             for f in [(0, 0, 0), (0, 0.5, 0.5), (0.5, 0, 0.5), (0.5, 0.5, 0)]],
+    "z": [(0, 0, 0), (0.5, 0.5, 0), (0, 0, 0.5), (0.5, 0.5, 0.5),
+          (0.25, 0.25, 0.75), (0.25, 0.75, 0.75), (0.75, 0.25, 0.75), (0.75, 0.75, 0.75),
+          (0.5, 1 / 6, 0.25), (0.5, 5 / 6, 0.25), (0, 0.5, 0.75), (0, 1 / 3, 0.25), (0, 2 / 3, 0.25),
+          (0.5, 0, 0.75)],
 }
+
+# block edges in units of the x edge
+SHAPE = {"z": (1.0, 3.0 ** 0.5, 1.0)}
 
 
 def _polygon_area(p):
@@ -49,24 +59,26 @@ def _polygon_area(p):
 
 
 def build(name, a):
-    """Cells of the block [0, a)^3: dict(pos (V,3), vol (V,), faces: list per cell of
+    """Cells of the block of x edge a (edges a * SHAPE[name], a cube by default): dict(pos (V,3),
+    vol (V,), faces: list per cell of
     (dx, dy, dz, target, hodge)).  Works from the Voronoi vertices of the atoms (the
     circumcentres of the Delaunay cells, cospherical groups already merged): a cell is the
     convex hull of the atoms nearest its vertex, two cells share a face when they share three
     or more atoms, and the face is the polygon of the shared atoms."""
     frac = np.array(ATOMS[name], dtype=np.float64)
+    A = a * np.array(SHAPE.get(name, (1.0, 1.0, 1.0)))  # block edges
     shifts = np.array([(i, j, k) for i in range(-2, 3) for j in range(-2, 3) for k in range(-2, 3)], float)
-    pts = ((frac[None, :, :] + shifts[:, None, :]).reshape(-1, 3)) * a
+    pts = ((frac[None, :, :] + shifts[:, None, :]).reshape(-1, 3)) * A
     tol = 1e-7 * a
     vor = Voronoi(pts)
     verts = vor.vertices
     # unique vertices (Qhull may repeat a vertex of a cospherical group)
-    verts = verts[np.all(np.abs(verts) < 4.0 * a, axis=1)]  # drop hull-artefact vertices far out
+    verts = verts[np.all(np.abs(verts) < 4.0 * A, axis=1)]  # drop hull-artefact vertices far out
     key = np.round(verts / (1e3 * tol)).astype(np.int64)
     _, first = np.unique(key, axis=0, return_index=True)
     verts = verts[np.sort(first)]
     # keep the vertices around the centre block: the central cells and their neighbours
-    near = np.all((verts > -0.6 * a) & (verts < 1.6 * a), axis=1)
+    near = np.all((verts > -0.6 * A) & (verts < 1.6 * A), axis=1)
     verts = verts[near]
     tree = cKDTree(pts)
     atoms, vols = [], []
@@ -76,22 +88,22 @@ def build(name, a):
         atoms.append(frozenset(int(t) for t in sel))
         vols.append(ConvexHull(pts[sel]).volume)
     vols = np.array(vols)
-    inside = np.all((verts >= -tol) & (verts < a - tol), axis=1)
+    inside = np.all((verts >= -tol) & (verts < A - tol), axis=1)
     central = np.nonzero(inside)[0]
     central = central[np.lexsort((verts[central, 2], verts[central, 1], verts[central, 0]))]
     ctree = cKDTree(verts[central])
 
     def ident(c):
         p = verts[c]
-        d = np.floor((p + tol) / a)
-        dist, i = ctree.query(p - d * a)
+        d = np.floor((p + tol) / A)
+        dist, i = ctree.query(p - d * A)
         return (int(i) if dist < 1e-5 * a else None), d.astype(int)
 
     vtree = cKDTree(verts)
     out_faces = []
     for c in central:
         row = []
-        for w in vtree.query_ball_point(verts[c], 1.05 * a):
+        for w in vtree.query_ball_point(verts[c], 1.05 * float(A.max())):
             if w == c:
                 continue
             common = atoms[c] & atoms[w]
@@ -115,12 +127,12 @@ def compare(name, table):
     max_hodge_diff, max_volume_diff) with table a spacetime_amd.lattice.Lattice."""
     a = table.block_width
     b = build(name, a)
-    m = match(b, table.pos, a)
+    m = match(b, table.pos, table.block_dims)
     if m is None:
         return dict(topology=False, max_hodge_diff=np.inf, max_volume_diff=np.inf)
     t, perm = m
     inv = {int(p): i for i, p in enumerate(perm)}
-    blk = np.floor((b["pos"][perm] + t) / a + 1e-9)  # block of each rebuilt cell after the translation
+    blk = np.floor((b["pos"][perm] + t) / table.block_dims + 1e-9)  # block of each rebuilt cell after the translation
     topo, hmax = True, 0.0
     for i in range(table.V):
         fs, fe = table.face_start[i], table.face_start[i + 1]
@@ -139,17 +151,18 @@ def compare(name, table):
     return dict(topology=topo, max_hodge_diff=hmax, max_volume_diff=float(np.abs(b["vol"][perm] - table.vol).max()))
 
 
-def match(built, table_pos, a, tol=1e-6):
-    """Translation t and permutation perm with built['pos'][perm[i]] + t == table_pos[i] (mod a),
-    or None."""
+def match(built, table_pos, edges, tol=1e-6):
+    """Translation t and permutation perm with built['pos'][perm[i]] + t == table_pos[i] (mod the
+    block edges), or None."""
     P = np.asarray(table_pos, float)
     B = built["pos"]
+    A = np.broadcast_to(np.asarray(edges, float), (3,))
     for j in range(len(B)):
         t = P[0] - B[j]
         perm = []
         for p in P:
-            diff = (B + t - p + 0.5 * a) % a - 0.5 * a
-            hit = np.nonzero(np.all(np.abs(diff) < tol * a, axis=1))[0]
+            diff = (B + t - p + 0.5 * A) % A - 0.5 * A
+            hit = np.nonzero(np.all(np.abs(diff) < tol * A, axis=1))[0]
             if len(hit) != 1:
                 break
             perm.append(int(hit[0]))

@@ -19,7 +19,7 @@ class OrcLattice(ctypes.Structure):
     _fields_ = [
         ("V", ctypes.c_int), ("nfaces", ctypes.c_int),
         ("face_start", _ip), ("face_d", _ip), ("face_tgt", _ip), ("face_hodge", _dp),
-        ("pos", _dp), ("vol", _dp), ("block_width", ctypes.c_double),
+        ("pos", _dp), ("vol", _dp), ("block_width", ctypes.c_double * 3),
     ]
 
 
@@ -119,7 +119,7 @@ class Oracle:
         s.face_hodge = _ptr(arrs["face_hodge"], _dp)
         s.pos = _ptr(arrs["pos"], _dp)
         s.vol = _ptr(arrs["vol"], _dp)
-        s.block_width = lat.block_width
+        s.block_width[:] = [float(x) for x in lat.block_dims]
         return s
 
     @staticmethod
@@ -356,7 +356,7 @@ class Oracle:
 
 
 def box_dims(lat, cells, edge):
-    """p0 centring used throughout: box of `cells` blocks of physical edge `edge` centred at 0."""
+    """p0 centring used throughout: box of `cells` blocks of physical x edge `edge` centred at 0."""
     s = edge / lat.block_width
-    p0 = [-0.5 * edge * c for c in cells]
+    p0 = [-0.5 * s * e * c for e, c in zip(lat.block_dims, cells)]
     return s, p0

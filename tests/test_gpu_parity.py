@@ -48,10 +48,10 @@ def _run_both(oracle, cfg, nsteps, field=None, seed=1, traversal=None):
     return (g0, g1), (b, a)
 
 
-@pytest.mark.parametrize("lattice_name", ["cube", "bcc", "fcc", "a15", "c15"])
+@pytest.mark.parametrize("lattice_name", ["cube", "bcc", "fcc", "a15", "c15", "z"])
 @pytest.mark.parametrize("ncomp", [1, 3, 5])
 def test_fp64_sweeps_match_oracle(oracle, lattice_name, ncomp):
-    dims = (5, 4, 3) if lattice_name != "c15" else (3, 2, 2)
+    dims = (5, 4, 3) if lattice_name not in ("c15", "z") else (3, 2, 2)
     cfg = _config(lattice_name, ncomp, gpe.FP64, dims)
     (g0, g1), (o0, o1) = _run_both(oracle, cfg, 12, field=FIELD if ncomp > 1 else None)
     assert rel_l2(g0, o0) < 1e-12
@@ -59,11 +59,11 @@ def test_fp64_sweeps_match_oracle(oracle, lattice_name, ncomp):
 
 
 @pytest.mark.parametrize("xpl", [1, 2])
-@pytest.mark.parametrize("lattice_name", ["cube", "bcc", "fcc", "a15", "c15"])
+@pytest.mark.parametrize("lattice_name", ["cube", "bcc", "fcc", "a15", "c15", "z"])
 @pytest.mark.parametrize("ncomp", [1, 3, 5])
 def test_fp32_sweeps_match_oracle(oracle, lattice_name, ncomp, xpl):
     """Both FP32 sweep bodies: one x per lane and x pairs per lane (odd nx: a half pair)."""
-    dims = (5, 4, 3) if lattice_name != "c15" else (3, 2, 2)
+    dims = (5, 4, 3) if lattice_name not in ("c15", "z") else (3, 2, 2)
     cfg = _config(lattice_name, ncomp, gpe.FP32, dims)
     (g0, g1), (o0, o1) = _run_both(oracle, cfg, 12, field=FIELD if ncomp > 1 else None, traversal=(8, 1, xpl))
     assert rel_l2(g0, o0) < 2e-5
@@ -87,6 +87,7 @@ def test_wide_rows_span_several_waves(oracle, precision, tol, xpl, nx):
     ("a15", 1, (33, 5, 2), (4, 1, 2)),  # 2 rows per wave over 33 values (17 lanes of 32)
     ("cube", 5, (32, 9, 3), (8, 1, 2)),  # 4 rows per wave; three rows of the last group past ny
     ("c15", 3, (6, 3, 2), (2, 1, 2)),
+    ("z", 5, (7, 4, 2), (4, 1, 2)),      # the non-cubic block, 4-row groups
     ("bcc", 5, (130, 5, 2), (4, 1, 2)),  # 4-row groups over 5 waves; the second tile's group past ny
     ("a15", 3, (82, 6, 3), (4, 0, 2)),   # 4-row groups of 41 pairs a row: 3 waves, rows split mid-wave
 ])
@@ -280,7 +281,7 @@ def _hash_uniform(seed, idx, s):
     return (z >> np.uint64(11)).astype(np.float64) / 2.0 ** 53
 
 
-@pytest.mark.parametrize("lattice_name,ncomp,precision", [("bcc", 5, 1), ("c15", 3, 1), ("fcc", 5, 0)])
+@pytest.mark.parametrize("lattice_name,ncomp,precision", [("bcc", 5, 1), ("c15", 3, 1), ("fcc", 5, 0), ("z", 3, 1)])
 def test_device_thomas_fermi_fill(lattice_name, ncomp, precision):
     from spacetime_amd import states
     cfg = _config(lattice_name, ncomp, precision, (6, 5, 4), cell=0.5)

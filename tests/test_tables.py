@@ -14,7 +14,7 @@ GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "gpblocks_scalar_bcc.
 @pytest.mark.parametrize("name", lattice.LATTICES)
 def test_table_shape_and_symmetry(name):
     lat = lattice.get(name)
-    expected_v = {"cube": 1, "bcc": 12, "fcc": 12, "a15": 46, "c15": 136}[name]
+    expected_v = {"cube": 1, "bcc": 12, "fcc": 12, "a15": 46, "c15": 136, "z": 80}[name]
     assert lat.V == expected_v
     assert np.all(np.abs(lat.face_d) <= 1)
     assert np.all((lat.face_tgt >= 0) & (lat.face_tgt < lat.V))
@@ -30,7 +30,8 @@ def test_table_shape_and_symmetry(name):
 @pytest.mark.parametrize("name", lattice.LATTICES)
 def test_volumes(name):
     lat = lattice.get(name)
-    assert abs(lat.vol.sum() - lat.block_width ** 3) < 1e-10 * lat.block_width ** 3
+    block = float(np.prod(lat.block_dims))
+    assert abs(lat.vol.sum() - block) < 1e-10 * block
     # the reference's single VOLUME constant is the volume of unknown 0's body
     assert abs(lat.vol[0] - lat.ref_volume) < 1e-12 * lat.ref_volume
     if name in ("cube", "bcc"):
@@ -106,7 +107,7 @@ def test_packed_faces_roundtrip():
         assert np.array_equal(p >> 8, lat.face_tgt)
 
 
-@pytest.mark.parametrize("name", ["cube", "bcc", "fcc", "a15", "c15"])
+@pytest.mark.parametrize("name", ["cube", "bcc", "fcc", "a15", "c15", "z"])
 def test_kinetic_bound_bounds_the_bloch_spectrum(name):
     """Lattice.kinetic_bound (Gershgorin) is >= every Bloch eigenvalue of -1/2 L, and is the
     exact maximum 6/s^2 on BCC (spectrum [0, 6/h^2])."""
@@ -121,7 +122,7 @@ def test_kinetic_bound_bounds_the_bloch_spectrum(name):
         assert bound == pytest.approx(6.0 / s ** 2)
 
 
-@pytest.mark.parametrize("name", ["cube", "bcc", "fcc", "a15", "c15"])
+@pytest.mark.parametrize("name", ["cube", "bcc", "fcc", "a15", "c15", "z"])
 def test_tables_match_an_independent_dec_construction(name):
     """The converted reference tables are the DEC discretisation of the lattice they name,
     rebuilt here from the textbook atom positions without the reference's builder: unknowns at
