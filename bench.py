@@ -40,11 +40,13 @@ METRIC = "node-updates/s on 1 GPU, FP32-form vs FP64 at ≤1e-4 rel L2; % HBM ro
 PEAK_HBM_GBS = 8000.0
 
 # HBM bytes per launch of the FP32 sweep kernel (traversal 16 rows, XCD remap, two x per lane),
-# measured with rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE on this workload
-# (profiles/k_sweep_spin1_bcc_128.md); FETCH_SIZE is scaled by 2.000 and WRITE_SIZE by 1.000,
-# the ratios of bytes moved to counter readings for the 1 GiB streaming copy in the same runs.
+# measured with rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE on each workload
+# (profiles/k_sweep_spin1_bcc_128.md, profiles/k_sweep_scalar_bcc_192.md); FETCH_SIZE is
+# scaled by 2.000 and WRITE_SIZE by 1.000, the ratios of bytes moved to counter readings for
+# the 1 GiB streaming copy in the same runs.
 PMC_TRAFFIC = {
     "spin1_bcc_128^3": 597935.1 * 1024 * 2.000 + 589824.0 * 1024 * 1.000,
+    "scalar_bcc_192^3": 673376.3 * 1024 * 2.000 + 663552.0 * 1024 * 1.000,
 }
 
 
