@@ -499,21 +499,21 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
                 out[e][c] = o;
             }
         }
+        // one 16-byte store per live lane: the lone last x of an odd-length line also writes its
+        // partner slot x = nx, which is line padding (lines have an even length), never read as a
+        // value -- every load of it is masked by a select and every reduction and copy to the ABI
+        // stops at nx.  (A separate 8-byte store for that lane made every wave run both store
+        // paths: odd-length lines ran 8-10% slower.)
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             float2 *dst = nxt + own_line + c * cstride + lo;
-            if (act1) {
+            if (act0) {
                 f4a o;
                 o.x = out[0][c].x;
                 o.y = out[0][c].y;
                 o.z = out[1][c].x;
                 o.w = out[1][c].y;
                 __builtin_nontemporal_store(o, (f4a *)dst);
-            } else if (act0) {
-                f2n o;
-                o.x = out[0][c].x;
-                o.y = out[0][c].y;
-                __builtin_nontemporal_store(o, (f2n *)dst);
             }
         }
         if constexpr (MODE == MODE_ITP) part *= a.tb.dv[v];
