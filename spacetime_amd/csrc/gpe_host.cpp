@@ -304,6 +304,7 @@ SweepArgs sweep_args(gpe_ctx *c, int mode_coef_sweep, const void *cur, void *nex
     a.prev = nullptr;
     a.fields = c->d_fields;
     a.sweep = mode_coef_sweep;
+    a.order = 0;
     a.quad = quad_active(c) ? 1 : 0;
     a.loss = c->cfg.alpha != 0.0 ? 1 : 0;
     a.coef = coef;
@@ -650,6 +651,7 @@ int gpe_step(gpe_ctx *c, int nsteps)
         HIPCALL(c, hipEventRecord(c->ev0, c->stream));
         for (int k = 0; k < n; ++k) {
             SweepArgs a = sweep_args(c, k, c->level[0], c->level[1], coef);
+            a.order = (int)(c->sweeps & 1);
             hipError_t e = launch_sweep(c->fp64, c->cfg.ncomp, MODE_UPDATE, a, c->stream);
             if (e != hipSuccess) return hip_fail(c, e, "sweep launch");
             std::swap(c->level[0], c->level[1]);
@@ -690,6 +692,7 @@ int gpe_itp(gpe_ctx *c, int nsteps, double dt_itp)
     for (int i = 0; i < nsteps; ++i) {
         SweepArgs a = sweep_args(c, 0, c->level[0], c->level[1], dt_itp);
         a.scale_in = scale[i & 1];
+        a.order = i & 1;
         e = launch_sweep(c->fp64, c->cfg.ncomp, MODE_ITP, a, c->stream);
         if (e == hipSuccess) e = launch_finalize(c->d_partials, c->gs.blocks, 1, -1, norm, c->d_red, c->stream);
         if (e == hipSuccess) e = launch_norm_to_scale(norm, 1.0, scale[(i + 1) & 1], c->stream);
@@ -917,6 +920,7 @@ int gpe_adjoint(gpe_ctx *c, const gpe_adjoint_spec *spec, gpe_adjoint_result *re
     ADJ_CHECK(hipMemcpyAsync(ck[1], c->level[0], lbytes, hipMemcpyDeviceToDevice, c->stream), "checkpoint");
     for (int k = 1; k < N; ++k) {  // produces psi_{k+1}
         SweepArgs a = sweep_args(c, k, c->level[0], c->level[1], 2.0 * sdt);
+        a.order = k & 1;
         ADJ_CHECK(launch_sweep(c->fp64, c->cfg.ncomp, MODE_UPDATE, a, c->stream), "forward sweep");
         std::swap(c->level[0], c->level[1]);
         const int m = k + 1;
@@ -1015,6 +1019,7 @@ int gpe_adjoint(gpe_ctx *c, const gpe_adjoint_spec *spec, gpe_adjoint_result *re
         for (int m = klo + 1; m < khi; ++m) {  // psi_{m+1} = psi_{m-1} - 2i dt H psi_m
             SweepArgs a = sweep_args(c, m, seg[m - klo], seg[m + 1 - klo], 2.0 * sdt);
             a.prev = seg[m - 1 - klo];
+            a.order = m & 1;
             ADJ_CHECK(launch_sweep(c->fp64, c->cfg.ncomp, MODE_UPDATE, a, c->stream), "recompute");
         }
         int qhi = khi, nq = 0;  // pending sweeps qhi, qhi - 1, ... in slots 0, 1, ...
@@ -1239,6 +1244,7 @@ int gpe_ensemble_step(gpe_ensemble *e, int nsteps, int graph_sweeps)
         const double coef = 2.0 * sweep_dt(c);
         e->h[i].args[0] = sweep_args(c, 0, c->level[0], c->level[1], coef);
         e->h[i].args[1] = sweep_args(c, 0, c->level[1], c->level[0], coef);
+        e->h[i].args[1].order = 1;
     }
     ENSCALL(e, hipMemcpyAsync(e->d, e->h.data(), sizeof(EnsembleMember) * e->h.size(), hipMemcpyHostToDevice,
                               e->stream));
@@ -1402,6 +1408,7 @@ int gpe_autotune(gpe_ctx *c, int trial_sweeps, int *tile_rows, int *xcd_remap, i
                 for (int k = 0; k < trial_sweeps + 2; ++k) {
                     if (k == 2) HIPCALL(c, hipEventRecord(c->ev0, c->stream));
                     SweepArgs a = sweep_args(c, 0, cur, nxt, 2.0 * sweep_dt(c));
+                    a.order = k & 1;
                     hipError_t e = launch_sweep(c->fp64, c->cfg.ncomp, MODE_UPDATE, a, c->stream);
                     if (e != hipSuccess) return hip_fail(c, e, "autotune sweep");
                     std::swap(cur, nxt);

@@ -119,6 +119,7 @@ struct SweepArgs {
     const void *prev;         // update: psi_{k-1} read from here instead of next (may be null)
     const FieldRec *fields;   // field of sweep k at fields[k]
     int sweep;                // index into fields
+    int order;                // 1: the workgroups take their tiles in reverse order (sweep_order)
     int quad, loss;           // quadratic Zeeman / three-body loss active
     double coef;              // update: next += coef * (-i) H cur ; ITP: dt
     const double *scale_in;   // ITP: scale applied to cur before H (device scalar)

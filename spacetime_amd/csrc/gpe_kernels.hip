@@ -543,7 +543,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, int sweep, int b,
 template <typename R, int C, int DEG, int MODE, int X>
 __global__ __launch_bounds__(256) void k_sweep(SweepArgs a)
 {
-    sweep_body<R, C, DEG, MODE, X>(a, a.sweep, blockIdx.x, gridDim.x);
+    sweep_body<R, C, DEG, MODE, X>(a, a.sweep, sweep_order(a.g, blockIdx.x, gridDim.x, a.order), gridDim.x);
 }
 
 // One leapfrog sweep of every member of an ensemble group in a single launch (a group: the
@@ -568,7 +568,7 @@ __global__ __launch_bounds__(256) void k_ensemble_sweep(EnsembleArgs e)
     if (local >= m.blocks) return;  // padding workgroup (uniform)
     const SweepArgs &a = m.args[e.parity];
     const int sweep = e.sweep + (e.sweep_base ? *e.sweep_base : 0);
-    sweep_body<R, C, DEG, MODE_UPDATE, X>(a, sweep, local, m.blocks);
+    sweep_body<R, C, DEG, MODE_UPDATE, X>(a, sweep, sweep_order(a.g, local, m.blocks, a.order), m.blocks);
 }
 
 __global__ void k_advance_base(int *base, int by)
