@@ -23,7 +23,8 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs a)
 {
     using V2 = vec2_t<R>;
     const Geometry &g = a.g;
-    const WaveCoord wc = decode(g);
+    const int b = sweep_order(g, blockIdx.x, gridDim.x, a.order);
+    const WaveCoord wc = decode(g, b, gridDim.x);
     double part[ADJ_COUNT];
 #pragma unroll
     for (int i = 0; i < ADJ_COUNT; ++i) part[i] = 0.0;
@@ -252,7 +253,7 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs a)
     __syncthreads();
     if (threadIdx.x < ADJ_COUNT) {
         const int i = threadIdx.x;
-        a.partials[(long long)blockIdx.x * ADJ_COUNT + i] = (red[0][i] + red[1][i]) + (red[2][i] + red[3][i]);
+        a.partials[(long long)b * ADJ_COUNT + i] = (red[0][i] + red[1][i]) + (red[2][i] + red[3][i]);
     }
 }
 

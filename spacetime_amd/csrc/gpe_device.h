@@ -54,13 +54,17 @@ __device__ __forceinline__ int logical_block(int b, int nb)
 // b mod 8, so logical_block of the result runs backwards through the same XCD's range).
 // Alternating the order between consecutive sweeps starts each sweep on the lines the one
 // before it touched last, which the memory-side cache still holds.
-__device__ __forceinline__ int sweep_order(const Geometry &g, int b, int nb, int order)
+__device__ __forceinline__ int reverse_in_xcd(int b, int nb)
 {
-    if (!order) return b;
-    if (!g.xcd_remap) return nb - 1 - b;
     const int q = nb >> 3, r = nb & 7, k = b & 7;
     const int len = q + (k < r ? 1 : 0);  // workgroups b' = k + 8 i, i < len
     return k + 8 * (len - 1 - (b >> 3));
+}
+
+__device__ __forceinline__ int sweep_order(const Geometry &g, int b, int nb, int order)
+{
+    if (!order) return b;
+    return g.xcd_remap ? reverse_in_xcd(b, nb) : nb - 1 - b;
 }
 
 struct WaveCoord {

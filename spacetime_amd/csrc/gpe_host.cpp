@@ -259,16 +259,16 @@ bool finish_geometry(Geometry &g)
     g.pairs = (g.nx + 1) / 2;
     // x pairs: one row per wave leaves lanes idle unless a line holds a multiple of 128 values.
     // Lines of <= 64 (<= 32) values: the wave's pair slots spread over a group of 2 (4) rows of
-    // the tile (a group size that divides the tile's rows).  Unit cells of <= 12 unknowns with
-    // lines of <= 128 values: the group of 1-8 rows with the best lane use when it fills at
-    // least 5% more lanes (BCC 70^3 +9%, FCC 45^3 +7%).  Larger cells (A15, C15) lose 3-5% with
-    // such groups at 70-90 blocks: their v-lines' shared neighbour lines are then used further
-    // apart in the traversal.
+    // the tile (a group size that divides the tile's rows).  Unit cells of <= 12 unknowns: the
+    // group of 1-8 rows with the best lane use when it fills at least 5% more lanes (BCC 70^3
+    // +9%, FCC 45^3 +7%, scalar FCC 160^3 +17%, spin-2 Cube 293^3 +8%, spin-1 BCC 192^3 +3%;
+    // scalar Cube 293^3 -2%).  Larger cells (A15, C15) lose 3-5% with such groups at 70-90
+    // blocks: their v-lines' shared neighbour lines are then used further apart in the traversal.
     int rows = 1;
     if (g.xpl == 2) {
         const int want = g.nx <= 32 ? 4 : (g.nx <= 64 ? 2 : 1);
         while (rows < want && g.ty % (2 * rows) == 0) rows *= 2;
-        if (g.V <= 12 && g.nx <= 128) {
+        if (g.V <= 12) {
             double best = lane_use(rows, g.pairs) + 0.05;
             for (int r = 2; r <= 8; r *= 2)
                 if (g.ty % r == 0 && lane_use(r, g.pairs) > best + 1e-9) {
@@ -1033,6 +1033,7 @@ int gpe_adjoint(gpe_ctx *c, const gpe_adjoint_spec *spec, gpe_adjoint_result *re
             aa.mu2 = A;
             aa.fields = c->d_fields;
             aa.sweep = k;
+            aa.order = k & 1;
             aa.quad = quad_active(c);
             aa.coef = 2.0 * sdt;
             aa.partials = d_qpart + sweep_part * nq;
@@ -1171,14 +1172,17 @@ int gpe_ensemble_create(gpe_ctx *const *members, int n, gpe_ensemble **out)
     e->per = c0->cfg.scheme == GPE_SCHEME_STAGGERED ? 2 : 1;
     e->h.resize(n);
     int begin = 0;
+    bool prev_solo = false;
     for (int i = 0; i < n; ++i) {
         const gpe_ctx *c = e->m[i];
         const int form = lane_form(c->gs.xpl, c->gs.rows);
-        if (i == 0 || c->gs.deg != e->groups.back().deg || form != e->groups.back().form ||
+        const bool solo = c->gs.blocks >= ENS_SOLO_BLOCKS;
+        if (i == 0 || solo || prev_solo || c->gs.deg != e->groups.back().deg || form != e->groups.back().form ||
             e->groups.back().count == ENS_GROUP_MAX) {
             e->groups.push_back(EnsembleGroup{c->gs.deg, form, i, 0, 0});
             begin = 0;
         }
+        prev_solo = solo;
         e->groups.back().count += 1;
         e->h[i].begin = begin;
         e->h[i].blocks = c->gs.blocks;
@@ -1244,7 +1248,6 @@ int gpe_ensemble_step(gpe_ensemble *e, int nsteps, int graph_sweeps)
         const double coef = 2.0 * sweep_dt(c);
         e->h[i].args[0] = sweep_args(c, 0, c->level[0], c->level[1], coef);
         e->h[i].args[1] = sweep_args(c, 0, c->level[1], c->level[0], coef);
-        e->h[i].args[1].order = 1;
     }
     ENSCALL(e, hipMemcpyAsync(e->d, e->h.data(), sizeof(EnsembleMember) * e->h.size(), hipMemcpyHostToDevice,
                               e->stream));

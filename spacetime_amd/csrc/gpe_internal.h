@@ -157,6 +157,7 @@ struct AdjArgs {
     void *mu2;                // mu_{k+2} in, mu_k out
     const FieldRec *fields;
     int sweep;
+    int order;                // 1: reverse workgroup order (sweep_order)
     int quad;
     double coef;
     double *partials;         // ADJ_COUNT per workgroup
@@ -190,6 +191,10 @@ struct EnsembleMember {
 // Members per launch: their first workgroups travel in the kernel arguments, so the
 // workgroup -> member search reads the kernel-argument segment, not the member records.
 constexpr int ENS_GROUP_MAX = 128;
+// A member of at least this many workgroups fills the GPU by itself (16 per CU) and gets a
+// launch of its own: consecutive sweeps of one launch then meet their own lines in the
+// memory-side cache, which a batch shares among all its members.
+constexpr int ENS_SOLO_BLOCKS = 4096;
 
 struct EnsembleArgs {
     const EnsembleMember *members;

@@ -554,7 +554,9 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a)
 template <typename R, int C, int DEG, int X>
 __global__ __launch_bounds__(256) void k_ensemble_sweep(EnsembleArgs e)
 {
-    const int b = blockIdx.x;
+    // parity-1 sweeps take the whole launch in reverse (members included), keeping each
+    // workgroup's XCD, so a sweep starts on the member the previous one ended with
+    const int b = e.parity ? reverse_in_xcd(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     int lo = 0, hi = e.nmember - 1;
     while (lo < hi) {  // last member with begin <= b
         const int mid = (lo + hi + 1) >> 1;
@@ -568,7 +570,7 @@ __global__ __launch_bounds__(256) void k_ensemble_sweep(EnsembleArgs e)
     if (local >= m.blocks) return;  // padding workgroup (uniform)
     const SweepArgs &a = m.args[e.parity];
     const int sweep = e.sweep + (e.sweep_base ? *e.sweep_base : 0);
-    sweep_body<R, C, DEG, MODE_UPDATE, X>(a, sweep, sweep_order(a.g, local, m.blocks, a.order), m.blocks);
+    sweep_body<R, C, DEG, MODE_UPDATE, X>(a, sweep, local, m.blocks);
 }
 
 __global__ void k_advance_base(int *base, int by)

@@ -11,10 +11,12 @@ pytestmark = pytest.mark.gpu
 gpe = pytest.importorskip("spacetime_amd.gpe")
 
 
-def _members(precision=gpe.FP32, ncomp=5):
+def _members(precision=gpe.FP32, ncomp=5, large=False):
     from spacetime_amd import states
     specs = [("bcc", (9, 7, 5)), ("cube", (70, 6, 4)), ("fcc", (5, 5, 4)), ("a15", (3, 4, 3)), ("c15", (2, 2, 3)),
              ("bcc", (66, 3, 2))]
+    if large:  # 4224 workgroups: launched on its own between the batched groups
+        specs.insert(1, ("bcc", (128, 64, 22)))
     out = []
     rng = np.random.default_rng(0)
     for i, (lat, dims) in enumerate(specs):
@@ -37,7 +39,7 @@ def _run_single(cfg, psi, ramp, n):
 
 @pytest.mark.parametrize("graph", [0, 4])
 def test_ensemble_equals_individual_runs(graph):
-    mem = _members()
+    mem = _members(large=True)
     n = 12
     singles = [_run_single(cfg, psi, ramp, n) for cfg, psi, ramp in mem]
     steppers = []

@@ -90,6 +90,9 @@ def test_wide_rows_span_several_waves(oracle, precision, tol, xpl, nx):
     ("z", 5, (7, 4, 2), (4, 1, 2)),      # the non-cubic block, 4-row groups
     ("bcc", 5, (130, 5, 2), (4, 1, 2)),  # 4-row groups over 5 waves; the second tile's group past ny
     ("a15", 3, (82, 6, 3), (4, 0, 2)),   # 4-row groups of 41 pairs a row: 3 waves, rows split mid-wave
+    ("bcc", 1, (192, 4, 2), (4, 1, 2)),  # 192-value lines: 2-row groups fill all lanes (one row: 3 of 4)
+    ("fcc", 1, (160, 6, 2), (4, 1, 2)),  # 160-value lines: 4-row groups of 80 pairs, rows past ny
+    ("cube", 3, (293, 10, 2), (8, 1, 2)),  # 8-row groups of 147 pairs; rows 10-15 of tile 2 past ny
 ])
 def test_packed_rows_match_oracle(oracle, lattice_name, ncomp, dims, traversal):
     """x pairs spread over groups of 2-8 y rows (lines that one row per wave leaves partly idle)."""
