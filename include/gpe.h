@@ -189,8 +189,9 @@ int gpe_last_step_timing(const gpe_ctx *ctx, double *ms_per_sweep, double *bytes
 /* ---------------------------------------------------------------------- ensembles ---- */
 /* No counterpart in the reference (one executable per run there): many independent runs,
  * each an ordinary started context (own lattice, size, dt, couplings, ramp), advanced
- * together with ONE kernel launch per sweep for all members.  Members must share ncomp,
- * precision and scheme and live on the current device. */
+ * together: per sweep, one kernel launch for each group of members sharing face degree and
+ * lane form (a member that fills the GPU alone gets its own launch).  Members must share
+ * ncomp, precision and scheme and live on the current device. */
 typedef struct gpe_ensemble gpe_ensemble;
 
 int gpe_ensemble_create(gpe_ctx *const *members, int n, gpe_ensemble **out);

@@ -362,12 +362,14 @@ int gpe_create(const gpe_config *cfg, gpe_ctx **out)
     g.C = cfg->ncomp;
     g.xpl = 1;
     g.ty = cfg->ny < 8 ? cfg->ny : 8;
-    // x-lines start on 128-byte boundaries unless that pads a line by more than 1/8 (short lines:
-    // the padding is fetched and written in part with every line), then on 16 / 32 bytes (an
-    // even number of values, which keeps the x-pair accesses 16-byte aligned)
+    // x-lines start on 128-byte boundaries unless that pads a line by more than 1/20 (the padding
+    // is fetched and written in part with every line), then on 16 / 32 bytes (an even number of
+    // values, which keeps the x-pair accesses 16-byte aligned).  Spin-2 FP32, even against
+    // 128-byte lines: BCC 115^3 (11% padding) +8.5%, BCC 118^3 (8.5%) +5%, A15 76^3 (5.3%) +2.4%;
+    // lines padded by less ran 1-3% faster with the 128-byte start.
     const long long align = c->fp64 ? 8 : 16;
     g.lx = ((long long)cfg->nx + align - 1) / align * align;
-    if (8 * g.lx > 9 * (long long)cfg->nx) g.lx = ((long long)cfg->nx + 1) / 2 * 2;
+    if (20 * g.lx > 21 * (long long)cfg->nx) g.lx = ((long long)cfg->nx + 1) / 2 * 2;
     g.xcd_remap = 1;
     c->gs = g;
     c->gs.xpl = default_xpl(c->fp64, g.deg, cfg->ncomp, cfg->nx);
