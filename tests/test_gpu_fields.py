@@ -90,3 +90,25 @@ def test_uv_theta_spin1():
     th_ref = 0.5 * np.arctan2(-2 * np.sum(m * q, -1), np.sum(m * m, -1) - np.sum(q * q, -1))
     th_ref = np.where(th_ref < 0, th_ref + np.pi, th_ref)
     np.testing.assert_allclose(th, th_ref, atol=1e-12)
+
+
+def test_save_volume_matches_field_maps(tmp_path):
+    """The volume file of a spin-1 state carries the device field maps (float32) at the
+    unknowns' positions, zero below the density threshold."""
+    from spacetime_amd import states, vtk
+    cfg = _cfg(3)
+    psi = random_state(cfg.shape, 5)
+    psi[:, 1, 2, 3, :4] *= 1e-3  # a few unknowns below the density threshold
+    with gpe.Stepper(cfg) as st:
+        st.set_state(psi)
+        dens = st.local_fields(gpe.FIELD_DENSITY)[..., 0].reshape(-1)
+        spin = st.local_fields(gpe.FIELD_SPIN).reshape(-1, 3)
+        theta = st.local_fields(gpe.FIELD_UV_THETA)[..., 6].reshape(-1)
+        vtk.save_volume(st, tmp_path / "s.vtk")
+    got = vtk.read_volume(tmp_path / "s.vtk")
+    live = dens > vtk.DENSITY_THRESHOLD
+    assert 0 < live.sum() < live.size
+    np.testing.assert_array_equal(got["points"], states.positions(cfg).reshape(-1, 3).astype(np.float32))
+    np.testing.assert_array_equal(got["density"], dens.astype(np.float32))
+    np.testing.assert_array_equal(got["localAvgSpin"], np.where(live[:, None], spin, 0.0).astype(np.float32))
+    np.testing.assert_array_equal(got["theta"], np.where(live, theta, 0.0).astype(np.float32))
