@@ -93,6 +93,9 @@ def test_wide_rows_span_several_waves(oracle, precision, tol, xpl, nx):
     ("bcc", 1, (192, 4, 2), (4, 1, 2)),  # 192-value lines: 2-row groups fill all lanes (one row: 3 of 4)
     ("fcc", 1, (160, 6, 2), (4, 1, 2)),  # 160-value lines: 4-row groups of 80 pairs, rows past ny
     ("cube", 3, (293, 10, 2), (8, 1, 2)),  # 8-row groups of 147 pairs; rows 10-15 of tile 2 past ny
+    ("bcc", 1, (128, 5, 3), (4, 1, 2)),  # one component, one row per wave; tile 2 holds only row 4
+    ("a15", 1, (70, 6, 3), (2, 0, 2)),   # one component, 2-row tiles, no XCD remap
+    ("z", 1, (66, 9, 3), (8, 1, 2)),     # one component over 33 pairs; tile 2 holds only row 8
 ])
 def test_packed_rows_match_oracle(oracle, lattice_name, ncomp, dims, traversal):
     """x pairs spread over groups of 2-8 y rows (lines that one row per wave leaves partly idle)."""
