@@ -826,9 +826,14 @@ void orc_adjoint_sweep(const orc_lattice *L, const orc_params *p, double t, doub
     const int V = L->V, C = p->ncomp, F = (C - 1) / 2;
     const size_t nc = ncells(p) * V;
     const double inv_s2 = 1.0 / (p->block_scale * p->block_scale);
-    int x, y, z, v, c, f, k;
-    for (k = 0; k < 7; ++k) sens[k] = 0;
-    for (z = 0; z < p->nz; ++z)
+    /* planes in parallel; each plane's sensitivities are summed in its own slot and the slots
+     * in plane order afterwards, so the result does not depend on the thread count */
+    double *plane = (double *)calloc((size_t)7 * p->nz, sizeof(double));
+    int z, k;
+#pragma omp parallel for schedule(static)
+    for (z = 0; z < p->nz; ++z) {
+        int x, y, v, c, f, k2;
+        double *sp = plane + 7 * (size_t)z;
         for (y = 0; y < p->ny; ++y)
             for (x = 0; x < p->nx; ++x)
                 for (v = 0; v < V; ++v) {
@@ -858,22 +863,27 @@ void orc_adjoint_sweep(const orc_lattice *L, const orc_params *p, double t, doub
                     double im_mp = 0;
                     for (c = 0; c < C; ++c) im_mp += cmul(cconj(mu[c]), ps[c]).im;
                     if (t < p->trap_off_time)
-                        for (k = 0; k < 3; ++k) sens[4 + k] += p->trap[k] * r[k] * r[k] * im_mp;
+                        for (k2 = 0; k2 < 3; ++k2) sp[4 + k2] += p->trap[k2] * r[k2] * r[k2] * im_mp;
                     if (F > 0) {
                         const double s = (F == 1) ? 1.0 : -1.0;
                         double pb[3];
-                        for (k = 0; k < 3; ++k) {
+                        for (k2 = 0; k2 < 3; ++k2) {
                             double e[3] = {0, 0, 0};
                             cplx fp[5] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
-                            e[k] = 1.0;
+                            e[k2] = 1.0;
                             spin_dot_apply(F, e, ps, fp);
                             double im = 0;
                             for (c = 0; c < C; ++c) im += cmul(cconj(mu[c]), fp[c]).im;
-                            pb[k] = s * im;
-                            sens[k] += pb[k];
+                            pb[k2] = s * im;
+                            sp[k2] += pb[k2];
                         }
-                        sens[3] += (r[0] - p->center[0]) * pb[0] + (r[1] - p->center[1]) * pb[1] -
-                                   2.0 * (r[2] - p->center[2]) * pb[2];
+                        sp[3] += (r[0] - p->center[0]) * pb[0] + (r[1] - p->center[1]) * pb[1] -
+                                 2.0 * (r[2] - p->center[2]) * pb[2];
                     }
                 }
+    }
+    for (k = 0; k < 7; ++k) sens[k] = 0;
+    for (z = 0; z < p->nz; ++z)
+        for (k = 0; k < 7; ++k) sens[k] += plane[7 * (size_t)z + k];
+    free(plane);
 }

@@ -377,7 +377,9 @@ int gpe_create(const gpe_config *cfg, gpe_ctx **out)
         delete c;
         return GPE_EINVAL;
     }
-    if ((long long)(g.ny + 2) * g.V * g.C * g.lx >= (1ll << 31)) {  // 32-bit z-plane offsets
+    // 32-bit index arithmetic in the kernels: z-plane offsets ((ny + 2) V C lx elements, the
+    // per-face line offsets and row strides) and line numbers ((z ny + y) V + v, line_index)
+    if ((long long)(g.ny + 2) * g.V * g.C * g.lx >= (1ll << 31) || (long long)g.nz * g.ny * g.V >= (1ll << 31)) {
         delete c;
         return GPE_EINVAL;
     }
@@ -896,11 +898,14 @@ int gpe_adjoint(gpe_ctx *c, const gpe_adjoint_spec *spec, gpe_adjoint_result *re
     res->checkpoint_bytes = (double)lbytes * 2 * nck;
     res->segment_bytes = (double)lbytes * (K - 2 + 2);
     hipError_t e = hipSuccess;
+    // on a failure the buffers are freed and the levels hold an intermediate state: the
+    // context needs gpe_set_state (and gpe_start) before it steps again
 #define ADJ_CHECK(call, what)                 \
     do {                                      \
         e = (call);                           \
         if (e != hipSuccess) {                \
             release();                        \
+            c->level1_valid = false;          \
             return hip_fail(c, e, what);      \
         }                                     \
     } while (0)
@@ -912,7 +917,7 @@ int gpe_adjoint(gpe_ctx *c, const gpe_adjoint_spec *spec, gpe_adjoint_result *re
         return rc;
     }
     // ---- forward pass, keeping (psi_jK, psi_jK+1)
-    HIPCALL(c, hipEventRecord(c->ev0, c->stream));
+    ADJ_CHECK(hipEventRecord(c->ev0, c->stream), "adjoint");
     ADJ_CHECK(hipMemcpyAsync(ck[0], c->level[0], lbytes, hipMemcpyDeviceToDevice, c->stream), "checkpoint");
     {
         SweepArgs a = sweep_args(c, 0, c->level[0], c->level[1], sdt);
@@ -934,14 +939,14 @@ int gpe_adjoint(gpe_ctx *c, const gpe_adjoint_spec *spec, gpe_adjoint_result *re
     }
     c->sweeps = N;
     c->level1_valid = true;
-    HIPCALL(c, hipEventRecord(c->ev1, c->stream));
-    HIPCALL(c, hipEventSynchronize(c->ev1));
+    ADJ_CHECK(hipEventRecord(c->ev1, c->stream), "adjoint");
+    ADJ_CHECK(hipEventSynchronize(c->ev1), "adjoint");
     float fms = 0.f;
-    HIPCALL(c, hipEventElapsedTime(&fms, c->ev0, c->ev1));
+    ADJ_CHECK(hipEventElapsedTime(&fms, c->ev0, c->ev1), "adjoint");
     res->forward_ms = fms;
 
     // ---- objective and mu_N
-    HIPCALL(c, hipEventRecord(c->ev0, c->stream));
+    ADJ_CHECK(hipEventRecord(c->ev0, c->stream), "adjoint");
     ObjArgs oa;
     oa.g = c->g;
     oa.tb = tables(c);
@@ -970,8 +975,8 @@ int gpe_adjoint(gpe_ctx *c, const gpe_adjoint_spec *spec, gpe_adjoint_result *re
         oa.mode = OBJ_OVERLAP;
         ADJ_CHECK(launch_objective(c->fp64, c->cfg.ncomp, oa, c->stream), "overlap grad");
         double ov[2];
-        HIPCALL(c, hipMemcpyAsync(ov, c->d_scalars + 32, sizeof(ov), hipMemcpyDeviceToHost, c->stream));
-        HIPCALL(c, hipStreamSynchronize(c->stream));
+        ADJ_CHECK(hipMemcpyAsync(ov, c->d_scalars + 32, sizeof(ov), hipMemcpyDeviceToHost, c->stream), "adjoint");
+        ADJ_CHECK(hipStreamSynchronize(c->stream), "adjoint");
         J = ov[0] * ov[0] + ov[1] * ov[1];
     } else {
         gpe_observables_t o;
@@ -986,9 +991,9 @@ int gpe_adjoint(gpe_ctx *c, const gpe_adjoint_spec *spec, gpe_adjoint_result *re
             oa.mode = OBJ_EXPLICIT;
             ADJ_CHECK(launch_objective(c->fp64, c->cfg.ncomp, oa, c->stream), "explicit terms");
             ADJ_CHECK(launch_finalize(d_part, c->g.blocks, 4, -1, c->d_scalars + 40, c->d_red, c->stream), "explicit sums");
-            HIPCALL(c, hipMemcpyAsync(explicit_mom, c->d_scalars + 40, sizeof(explicit_mom), hipMemcpyDeviceToHost,
-                                      c->stream));
-            HIPCALL(c, hipStreamSynchronize(c->stream));
+            ADJ_CHECK(hipMemcpyAsync(explicit_mom, c->d_scalars + 40, sizeof(explicit_mom), hipMemcpyDeviceToHost,
+                                      c->stream), "adjoint");
+            ADJ_CHECK(hipStreamSynchronize(c->stream), "adjoint");
             // H psi_N at the time of psi_N into mu[1], then scaled by 2 dv
             int frc = upload_fields(c, N, 1);
             if (frc) {
@@ -1051,11 +1056,11 @@ int gpe_adjoint(gpe_ctx *c, const gpe_adjoint_spec *spec, gpe_adjoint_result *re
         }
     }
     std::vector<double> steps((size_t)ADJ_COUNT * N);
-    HIPCALL(c, hipMemcpyAsync(steps.data(), d_steps, sizeof(double) * steps.size(), hipMemcpyDeviceToHost, c->stream));
-    HIPCALL(c, hipEventRecord(c->ev1, c->stream));
-    HIPCALL(c, hipEventSynchronize(c->ev1));
+    ADJ_CHECK(hipMemcpyAsync(steps.data(), d_steps, sizeof(double) * steps.size(), hipMemcpyDeviceToHost, c->stream), "adjoint");
+    ADJ_CHECK(hipEventRecord(c->ev1, c->stream), "adjoint");
+    ADJ_CHECK(hipEventSynchronize(c->ev1), "adjoint");
     float bms = 0.f;
-    HIPCALL(c, hipEventElapsedTime(&bms, c->ev0, c->ev1));
+    ADJ_CHECK(hipEventElapsedTime(&bms, c->ev0, c->ev1), "adjoint");
     res->backward_ms = bms;
     release();
 
@@ -1327,14 +1332,20 @@ int gpe_ensemble_timing(const gpe_ensemble *e, double *ms_per_sweep, double *unk
 
 // ------------------------------------------------------------------------ autotuning ---
 
-static void set_traversal(gpe_ctx *c, int ty, int remap, int xpl)
+// Apply a traversal to both geometries; on failure (a workgroup count past the 2^30 limit of
+// the decode) both are left as they were and false is returned.
+static bool set_traversal(gpe_ctx *c, int ty, int remap, int xpl)
 {
-    c->gs.xpl = xpl;
-    for (Geometry *g : {&c->g, &c->gs}) {
-        g->ty = ty;
-        g->xcd_remap = remap;
-        (void)finish_geometry(*g);  // cannot overflow: at most ny + 31 rows per z-plane
+    Geometry g = c->g, gs = c->gs;
+    gs.xpl = xpl;
+    for (Geometry *p : {&g, &gs}) {
+        p->ty = ty;
+        p->xcd_remap = remap;
+        if (!finish_geometry(*p)) return false;
     }
+    c->g = g;
+    c->gs = gs;
+    return true;
 }
 
 // Reduction partials are sized by the workgroups of the one-x-per-lane traversal.
@@ -1357,7 +1368,8 @@ int gpe_set_traversal(gpe_ctx *c, int tile_rows, int xcd_remap, int x_per_lane)
     if (x_per_lane == 2 && !pair_lanes_ok(c->fp64, c->g.deg, c->cfg.ncomp))
         return fail(c, GPE_EINVAL, "two x per lane needs FP32");
     const int before = c->g.blocks;
-    set_traversal(c, tile_rows, xcd_remap, x_per_lane);
+    if (!set_traversal(c, tile_rows, xcd_remap, x_per_lane))
+        return fail(c, GPE_EINVAL, "traversal needs more workgroups than the decode supports");
     return grow_partials(c, before);
 }
 
@@ -1392,14 +1404,25 @@ int gpe_autotune(gpe_ctx *c, int trial_sweeps, int *tile_rows, int *xcd_remap, i
     }
     if (trial_sweeps > 0) {
         const size_t lbytes = (size_t)c->level_elems * c->elem_bytes;
-        void *s0 = nullptr, *s1 = nullptr;
-        double *parts = nullptr;
-        if (hipMalloc(&s0, lbytes) != hipSuccess || hipMalloc(&s1, lbytes) != hipSuccess) {
-            if (s0) (void)hipFree(s0);
+        // scratch copies of the two levels, freed when the trials end (also on failure), when
+        // the context's traversal on entry is restored too (the winner is applied below)
+        struct Trial {
+            gpe_ctx *c;
+            Geometry g, gs;
+            void *s0 = nullptr, *s1 = nullptr;
+            ~Trial()
+            {
+                if (s0 || s1) (void)hipStreamSynchronize(c->stream);
+                if (s0) (void)hipFree(s0);
+                if (s1) (void)hipFree(s1);
+                c->g = g;
+                c->gs = gs;
+            }
+        } trial{c, c->g, c->gs};
+        if (hipMalloc(&trial.s0, lbytes) != hipSuccess || hipMalloc(&trial.s1, lbytes) != hipSuccess)
             return fail(c, GPE_ENOMEM, "autotune scratch does not fit");
-        }
-        HIPCALL(c, hipMemcpyAsync(s0, c->level[0], lbytes, hipMemcpyDeviceToDevice, c->stream));
-        HIPCALL(c, hipMemcpyAsync(s1, c->level[1], lbytes, hipMemcpyDeviceToDevice, c->stream));
+        HIPCALL(c, hipMemcpyAsync(trial.s0, c->level[0], lbytes, hipMemcpyDeviceToDevice, c->stream));
+        HIPCALL(c, hipMemcpyAsync(trial.s1, c->level[1], lbytes, hipMemcpyDeviceToDevice, c->stream));
         int rc = upload_fields(c, c->sweeps, 1);
         if (rc) return rc;
         const int tys[] = {1, 2, 4, 8, 16, 32};
@@ -1408,8 +1431,8 @@ int gpe_autotune(gpe_ctx *c, int trial_sweeps, int *tile_rows, int *xcd_remap, i
         for (int ty : tys) {
             if (ty > c->g.ny && ty != 1) continue;
             for (int remap = 0; remap < 2; ++remap) {
-                set_traversal(c, ty, remap, xpl);
-                void *cur = s0, *nxt = s1;
+                if (!set_traversal(c, ty, remap, xpl)) continue;  // too many workgroups: not a candidate
+                void *cur = trial.s0, *nxt = trial.s1;
                 for (int k = 0; k < trial_sweeps + 2; ++k) {
                     if (k == 2) HIPCALL(c, hipEventRecord(c->ev0, c->stream));
                     SweepArgs a = sweep_args(c, 0, cur, nxt, 2.0 * sweep_dt(c));
@@ -1431,13 +1454,11 @@ int gpe_autotune(gpe_ctx *c, int trial_sweeps, int *tile_rows, int *xcd_remap, i
                 }
             }
         }
-        (void)hipFree(s0);
-        (void)hipFree(s1);
-        (void)parts;
         std::lock_guard<std::mutex> lk(g_tune_mutex);
         g_tune_cache[key] = std::make_tuple(best_ty, best_remap, best_xpl);
     }
-    set_traversal(c, best_ty, best_remap, best_xpl);
+    if (!set_traversal(c, best_ty, best_remap, best_xpl))
+        return fail(c, GPE_EINVAL, "traversal needs more workgroups than the decode supports");
     int rc = grow_partials(c, max_blocks_before);
     if (rc) return rc;
     if (tile_rows) *tile_rows = best_ty;

@@ -19,7 +19,7 @@ struct FastDiv {
 // Device layout of one time level: z, y, unknown v, component, and x fastest,
 //   element(z, y, v, c, x) = ((((z * ny + y) * V + v) * C + c) * lx + x
 // with lx >= nx rounded up so every x-line starts on a 128-byte boundary (or, where that
-// would pad a line by more than 1/8, lx = nx rounded up to even).  One wave
+// would pad a line by more than 1/20, lx = nx rounded up to even).  One wave
 // handles 64 consecutive x of one (z, y, v) line, so the unknown v, the face table row and
 // the neighbour rows are uniform across the wave and every neighbour load is a contiguous
 // 64-lane access.  Out-of-box neighbours read as zero (Dirichlet, as the reference's

@@ -61,3 +61,15 @@ def test_create_rejects_bad_config_without_device(lib):
     cfg = gpe.Config(lattice="bcc", ncomp=2).to_c()
     assert lib.gpe_create(ctypes.byref(cfg), ctypes.byref(h)) == -22
     assert not h.value
+
+
+def test_create_rejects_line_counts_past_32_bits_without_device(lib):
+    # the kernels number lines (z ny + y) V + v in 32 bits: a box with nz ny V >= 2^31 lines must
+    # be refused before any allocation (FP32 scalar BCC 2 x 13400 x 13400 blocks, ~69 GB, would
+    # otherwise fit on the device and index out of bounds)
+    from spacetime_amd import gpe
+    lib.gpe_create.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    h = ctypes.c_void_p()
+    cfg = gpe.Config(lattice="bcc", dims=(2, 13400, 13400), ncomp=1, precision=gpe.FP32).to_c()
+    assert lib.gpe_create(ctypes.byref(cfg), ctypes.byref(h)) == -22
+    assert not h.value

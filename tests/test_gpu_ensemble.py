@@ -149,3 +149,46 @@ def test_graph_replays_each_match_eager():
                 assert st.time() == ref.time()
     for s in steppers + singles:
         s.close()
+
+
+@pytest.mark.parametrize("graph", [0, 4])
+def test_mixed_lattice_fp32_spin2_ensemble_matches_oracle(oracle, graph):
+    """The configured ensemble's kind at test size: spin-2 FP32 members on the five lattices
+    (Cube, BCC, FCC, A15, C15) with seeded signs of c1, c2, their own block counts, dt and 8-knot
+    Bz ramps, stepped together; every member against the FP64 oracle stepped alone."""
+    from spacetime_amd import lattice, states
+    rng = np.random.default_rng(44)
+    specs = [("cube", (41, 6, 5)), ("bcc", (7, 6, 5)), ("fcc", (6, 5, 4)), ("a15", (4, 3, 3)), ("c15", (3, 2, 2)),
+             ("bcc", (70, 3, 2)), ("fcc", (33, 4, 3))]
+    n = 12
+    steppers, refs = [], []
+    for i, (lat, dims) in enumerate(specs):
+        signs = rng.choice([-1.0, 1.0], 2)
+        dt = float(rng.choice([1e-4, 2e-4]))
+        cfg = gpe.Config.centred_box(lat, dims, 0.7, ncomp=5, precision=gpe.FP32, dt=dt, c0=100.0,
+                                     c1=20.0 * signs[0], c2=5.0 * signs[1])
+        psi = normalized(random_state(cfg.shape, 30 + i), cfg)
+        knots = rng.uniform(-0.2, 0.2, 8)
+        b_segments = states.knot_ramp(knots, (n + 1) * dt)
+        q_segments = [(0.01, 1e30, 0)]
+        st = gpe.Stepper(cfg)
+        st.set_ramp(q_segments=q_segments, b_segments=b_segments)
+        st.set_state(psi)
+        st.start(gpe.START_EULER)
+        steppers.append(st)
+        L = oracle.lattice(lattice.get(lat))
+        p = oracle_params(oracle, cfg)
+        r = oracle.ramp(q_segments=q_segments, b_segments=b_segments)
+        bq0, bb0 = oracle.ramp_signal(r, 0.0)
+        with_field(p, bq=bq0, bb=bb0)
+        a = psi.copy()
+        b = oracle.forward_euler(L, p, a, cfg.dt)
+        oracle.leapfrog3(L, p, a, b, cfg.dt, n, t0=cfg.dt, ramp=r)
+        refs.append((b, a))
+    with gpe.Ensemble(steppers) as ens:
+        ens.step(n, graph_sweeps=graph)
+    for (lat, dims), st, (o0, o1) in zip(specs, steppers, refs):
+        assert st.time()[1] == n + 1
+        assert rel_l2(st.get_state(0), o0) < 2e-5, (lat, dims)
+        assert rel_l2(st.get_state(1), o1) < 2e-5, (lat, dims)
+        st.close()

@@ -351,3 +351,93 @@ def test_expanding_grid_matches_oracle(oracle, precision, tol):
             st.adjoint(4, 2, gpe.OBJ_MAGNETIZATION_Z)
     assert p.block_scale > 1.05 * s0
     assert rel_l2(got, b) < tol
+
+
+@pytest.mark.parametrize("xpl,nx", [(1, 7), (2, 7), (2, 66), (1, 66)])
+def test_fp32_staggered_matches_oracle(oracle, xpl, nx):
+    """The FP32 form of the scalar staggered scheme (odd += -i dt H(even); even += -i dt H(odd),
+    GPBlocks.cpp:560-606) with both lane widths, odd and even line lengths, phase start."""
+    from spacetime_amd import lattice
+    cfg = _config("bcc", 1, gpe.FP32, (nx, 5, 4), dt=1e-3, scheme=gpe.STAGGERED, c1=0.0, c2=0.0, q=0.0)
+    L = oracle.lattice(lattice.get("bcc"))
+    p = oracle_params(oracle, cfg)
+    even = random_state(cfg.shape, 6)
+    mu = 2.9
+    e, o = even.copy(), np.exp(1j * mu * cfg.dt / 2) * even
+    oracle.staggered(L, p, e, o, cfg.dt, 15)
+    with gpe.Stepper(cfg) as st:
+        st.set_traversal(4, 1, xpl)
+        st.set_state(even)
+        st.start(gpe.START_PHASE, mu)
+        st.step(15)
+        g_even, g_odd = st.get_state(0), st.get_state(1)
+    assert rel_l2(g_even, e) < 2e-5
+    assert rel_l2(g_odd, o) < 2e-5
+
+
+def _golden_cases():
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "gpblocks_scalar_bcc.json")
+    with open(path) as f:
+        return json.load(f)["cases"]
+
+
+@pytest.mark.parametrize("case", [0, 1])
+def test_staggered_run_matches_reference_golden(oracle, case):
+    """The HIP FP64 staggered scheme against the compiled reference's own output: the norm and
+    overlap error per iteration printed by GrossPitaevskii/GPBlocks.cpp integrateInTime
+    (:528-538, loop :560-606) for a stationary vortex state.  Box sizing (:456-460) and the
+    first odd level e^{i mu dt/2} even (:516-523) follow the reference; the vortex profile is
+    relaxed by the library (gpe_vortex_relax) and sampled at GPBlocks' own positions (the
+    reference interpolant is discontinuous in its last z row, so positions one ulp apart can
+    sample different rows); dt and the steps per iteration are the reference's rule as restated
+    by the oracle."""
+    from spacetime_amd import lattice, states
+    c = _golden_cases()[case]
+    lat = lattice.get("bcc")
+    w = lat.block_dims
+    s = c["block_scale"]
+    minp = np.array([-c["rmax"], -c["rmax"], -c["zmax"]])
+    maxp = -minp
+    dims = tuple(int((maxp[a] - minp[a]) / (s * w[a])) + 1 for a in range(3))
+    p0 = tuple(0.5 * (minp[a] + maxp[a] - s * w[a] * dims[a]) for a in range(3))
+    # the reference's dt rule (GPBlocks.cpp:497-511), through the oracle's restatement
+    vs = oracle.vortex(c["kappa"], c["g"], c["rmax"], c["zmax"], c["rstep"], c["zstep"], maxiters=c["maxiters"])
+    gl = __import__("copy").copy(lat)
+    gl.pos = np.array(c["bcc_pos"])
+    _, _, dt, steps = oracle.gpblocks_integrate(oracle.lattice(gl), vs, c["kappa"], c["g"], s, list(minp),
+                                                list(maxp), c["period"], 0)
+    # the library's own relaxation of the same profile, interpolated by the reference's rule
+    lib_vs = states.vortex_state(c["kappa"], c["g"], c["rmax"], c["zmax"], c["rstep"], c["zstep"],
+                                 maxiters=c["maxiters"])
+    assert lib_vs["mu"] == pytest.approx(c["vortex_mu"], rel=1e-12)
+    vs = dict(vs, f=lib_vs["f"])
+    # psi0 at GPBlocks' positions: p0 + s (w i + pos_v)
+    nz, ny, nx = dims[2], dims[1], dims[0]
+    iz, iy, ix = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    blk = np.stack([ix, iy, iz], -1)[:, :, :, None, :] * w[None, None, None, None, :]
+    r = np.array(p0)[None, None, None, None, :] + s * (blk + gl.pos[None, None, None, :, :])
+    rr = np.hypot(r[..., 0], r[..., 1])
+    fv = np.vectorize(lambda a, b: oracle.vortex_function(vs, a, b))(rr, r[..., 2])
+    with np.errstate(invalid="ignore", divide="ignore"):
+        ph = ((r[..., 0] + 1j * r[..., 1]) / rr) ** c["kappa"]
+    psi0 = np.where(rr < 1e-30, 0.0, fv * ph)[None].astype(np.complex128)
+    cfg = gpe.Config(lattice="bcc", ncomp=1, precision=gpe.FP64, scheme=gpe.STAGGERED, dims=dims, block_scale=s,
+                     p0=p0, dt=dt, c0=c["g"])
+    vol = s ** 3 * lat.vol[0]
+    normsq, error = [], []
+    with gpe.Stepper(cfg) as st:
+        st.set_state(psi0)
+        st.start(gpe.START_PHASE, c["vortex_mu"])
+        for it in range(c["iterations"] + 1):
+            even = st.get_state(0)
+            n = float(np.sum(np.abs(even) ** 2) * vol)
+            ov = complex(np.sum(vol * np.conj(psi0) * even))
+            normsq.append(n)
+            error.append(n - abs(ov))
+            if it < c["iterations"]:
+                st.step(steps)
+    np.testing.assert_allclose(normsq, c["normsq"], rtol=1e-12)
+    np.testing.assert_allclose(error, c["error"], rtol=0, atol=1e-12)
+    assert max(abs(e) for e in c["error"][1:]) > 1e-5
