@@ -446,35 +446,38 @@ def run_adjoint(args, log):
 
 
 def prepare_state(cfg, knots, itp_steps, itp_dt, log):
-    """FP64 ground state by imaginary time on the GPU, then the imprint / noise."""
-    from spacetime_amd import gpe, states
+    """FP64 ground state on the device: Thomas-Fermi fill at the unit-norm chemical potential,
+    imaginary time with renormalisation, then for spinors the polar projection and 0.5% spinor
+    noise (device kernels), for the scalar the vortex-ring imprint and 1% noise (host)."""
+    from spacetime_amd import gpe, lattice, states
     import dataclasses
-    r = states.positions(cfg)
-    tf, mu = states.thomas_fermi(cfg, cfg.c0, r=r)
-    psi = np.zeros(cfg.shape, dtype=np.complex128)
-    psi[cfg.ncomp // 2] = tf
-    del tf
+    lam3 = cfg.trap[0] * cfg.trap[1] * cfg.trap[2]
+    mu = 0.5 * (15.0 * cfg.c0 * lam3 / (4.0 * np.pi)) ** 0.4
     cfg64 = dataclasses.replace(cfg, precision=gpe.FP64, scheme=gpe.LEAPFROG3)
     # imaginary-time Euler contracts every mode only for dt * lambda_max < 2: keep dt * ||H|| <= 1
-    from spacetime_amd import lattice
     lam = lattice.get(cfg.lattice).kinetic_bound(cfg.block_scale) + mu
     if itp_dt * lam > 1.0:
         log("itp: dt %g lowered to %g (kinetic bound %.4g)" % (itp_dt, 1.0 / lam, lam))
         itp_dt = 1.0 / lam
+    weights = np.zeros(cfg.ncomp, dtype=np.complex128)
+    weights[cfg.ncomp // 2] = 1.0
     t0 = time.time()
     with gpe.Stepper(cfg64) as st:
-        st.set_state(psi)
+        st.fill_thomas_fermi(cfg.c0, mu, weights)
         st.itp(itp_steps, itp_dt)
-        psi = st.get_state(0)
         e = st.observables(0)
+        if cfg.ncomp > 1:
+            st.project("polar")
+            st.add_noise(0.005, 54363)
+            st.scale(1.0 / np.sqrt(st.observables(0).norm))
+        psi = st.get_state(0)
     log("itp: %d steps dt=%g in %.2fs, E=%.6f mu_TF=%.4f" % (itp_steps, itp_dt, time.time() - t0, e.energy, mu))
     if cfg.ncomp == 1:
+        r = states.positions(cfg)
         psi = psi * states.vortex_ring_phase(r, 2.0)[None]
+        del r
         psi = states.multiplicative_noise(psi, 0.01, 54363)
-    else:
-        psi = states.project(psi, "polar")
-        psi = states.spinor_noise(psi, 0.005, 54363)
-    psi *= np.sqrt(1.0 / states.norm(cfg, psi))
+        psi *= np.sqrt(1.0 / states.norm(cfg, psi))
     return psi
 
 

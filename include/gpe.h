@@ -220,6 +220,22 @@ int gpe_autotune(gpe_ctx *ctx, int trial_sweeps, int *tile_rows, int *xcd_remap,
 int gpe_set_traversal(gpe_ctx *ctx, int tile_rows, int xcd_remap, int x_per_lane);
 int gpe_traversal(const gpe_ctx *ctx, int *tile_rows, int *xcd_remap, int *x_per_lane);
 
+/* The launch a sweep of a configuration would make for a traversal, computed on the host
+ * without a device: workgroups of four waves, each wave a 64-lane segment of one (z, y, v)
+ * x-line (rows = 1) or of a group of `rows` y rows (x pairs spread over the rows), in the
+ * order y-tile, z, row (group) in tile, unknown v, segment; and the constants of the wave
+ * decode's divisions by invariants (d, m, l: n / d = (t + ((n - t) >> 1)) >> (l - 1), t the
+ * high 32 bits of n m; d = 1 passes n through).  No counterpart in the reference, whose
+ * launch shape is fixed (Spin1GpeBCC.cu:1376-1380). */
+typedef struct {
+    int nx, ny, nz, V, C;
+    int x_per_lane, rows, pairs, nseg, tile_rows, row_groups, ntile, xcd_remap;
+    long long lx, waves;
+    int blocks;
+    unsigned div_nseg[3], div_V[3], div_rows[3], div_nz[3], div_pairs[3]; /* d, m, l */
+} gpe_traversal_plan;
+int gpe_plan_traversal(const gpe_config *cfg, int tile_rows, int xcd_remap, int x_per_lane, gpe_traversal_plan *out);
+
 /* Achievable HBM rate on the current device, from streaming kernels over two buffers of
  * `bytes` each (16-byte non-temporal accesses), averaged over `reps` launches: a copy
  * (2 x bytes moved) and an in-place b += a / 2 (3 x bytes: the sweep's read-2-write-1 pattern).
@@ -273,7 +289,17 @@ typedef struct {
     int checkpoint_every; /* K >= 2: a (psi_jK, psi_jK+1) pair is kept every K sweeps     */
     int objective;        /* GPE_OBJ_*                                                    */
     const double *target; /* GPE_OBJ_OVERLAP: host state, ABI layout; else NULL           */
+    int fp64_trajectory;  /* FP32 context: 1 keeps psi and mu in FP64 (the run is copied to a
+                             temporary FP64 context and the final levels copied back); 0 FP32 */
 } gpe_adjoint_spec;
+
+/* Precision limit.  With FP32 storage of psi and mu each sweep rounds both to 6e-8; gradients
+ * that are small residues of cancelling per-sweep terms do not survive it.  Measured (spin-1
+ * BCC, c0 1e4, 4000 sweeps, overlap near its maximum): the trap-frequency gradients (~6e-7,
+ * per-sweep terms ~1e-2) come out 14-16% off from an FP32 context against central
+ * differences, and on the FP64 oracle rounding only the stored trajectory to FP32 moves them
+ * by 19-47%, rounding only mu by up to 2.7%; the ramp-knot gradients stay within 3e-5 either
+ * way.  fp64_trajectory = 1 gives the FP64 result (within 1e-4 there) on an FP32 context. */
 
 typedef struct {
     double objective;          /* J                                                      */
@@ -292,6 +318,103 @@ int gpe_adjoint(gpe_ctx *ctx, const gpe_adjoint_spec *spec, gpe_adjoint_result *
  * lambda_x,y,z of the reference's trap() (Spin1GpeBCC.cu:112-122), a parameter here so the
  * adjoint's trap gradients can be checked by finite differences. */
 int gpe_set_trap(gpe_ctx *ctx, const double trap[3]);
+
+/* ------------------------------------------------------------- context queries ---- */
+/* The configuration of a context (the trap as last set by gpe_set_trap). */
+int gpe_get_config(const gpe_ctx *ctx, gpe_config *out);
+/* Set the run clock without touching the levels: time t0 of level 0 when `sweeps` sweeps have
+ * been taken since then (gpe_time reports t0 + sweeps * sweep dt).  Restores a saved clock
+ * exactly on resume; the reference keeps its clock t in the state file names instead
+ * (Spin1GpeBCC.cu:1229,1690). */
+int gpe_set_time(gpe_ctx *ctx, double t0, long long sweeps);
+/* The clock gpe_set_time sets: t0 and the sweeps taken since (gpe_time gives t itself). */
+int gpe_clock(const gpe_ctx *ctx, double *t0, long long *sweeps);
+
+/* ------------------------------------------------------- spinor states on the device ---- */
+/* Project every unknown of a level onto a spinor phase, keeping its density n = |psi|^2:
+ *   spin-1 (m = 1, 0, -1)                       Spin1GpeBCC.cu:463-551
+ *     GPE_PHASE_POLAR     (0, sqrt n, 0)
+ *     GPE_PHASE_FERRO     (sqrt n, 0, 0)
+ *     GPE_PHASE_MIXED     (sqrt(n param), sqrt(n (1 - param)), 0)     param = ferro fraction
+ *   spin-2 (m = 2..-2)                          Spin2GpeBCC.cu:417-558
+ *     GPE_PHASE_UN            (0, 0, sqrt n, 0, 0)
+ *     GPE_PHASE_BN_HORIZONTAL (sqrt(n/2), 0, 0, 0, e^{i param} sqrt(n/2))
+ *     GPE_PHASE_BN_VERTICAL   (0, sqrt(n/2), 0, e^{i param} sqrt(n/2), 0)
+ *     GPE_PHASE_CYCLIC        (sqrt(n/3), 0, 0, e^{i param} sqrt(2n/3), 0)
+ *     GPE_PHASE_FERRO         (sqrt n, 0, 0, 0, 0)
+ * Replaces the reference's projection kernels, launched on its state before stepping. */
+#define GPE_PHASE_POLAR 0
+#define GPE_PHASE_FERRO 1
+#define GPE_PHASE_MIXED 2
+#define GPE_PHASE_UN 3
+#define GPE_PHASE_BN_HORIZONTAL 4
+#define GPE_PHASE_BN_VERTICAL 5
+#define GPE_PHASE_CYCLIC 6
+int gpe_project_spinor(gpe_ctx *ctx, int level, int phase, double param);
+/* psi_m += amplitude sqrt(n) eps_m on a level, eps_m uniform in the unit disk from a
+ * counter-based hash of (seed, unknown, component), n the density before the noise.  The
+ * reference draws normal variates on the host instead (Spin1GpeBCC.cu:1257-1292). */
+int gpe_add_noise(gpe_ctx *ctx, int level, double amplitude, unsigned long long seed);
+/* Piecewise-linear bias ramp Bz(t) through n >= 2 equally spaced knots over `duration` (the
+ * first knot held at t = 0, then linear segments; LINEAR segments of getSignal,
+ * AliceRingRamps.h:74-135) with a constant quadrupole gradient bq: gpe_set_ramp with those
+ * segments, time_scale 1 and unit scales. */
+int gpe_set_knot_ramp(gpe_ctx *ctx, int n, const double *bz_knots, double duration, double bq);
+
+/* ------------------------------------------------------- state files and checkpoints ---- */
+/* The reference solvers' state file: the raw padded block array they dump with
+ * ofstream::write (Spin1GpeBCC.cu:1690-1699, read back :1228-1237): (nz+2)(ny+2)(nx+2) blocks (a
+ * one-block zero halo), block (i, j, k) at index (k+1)(ny+2)(nx+2) + (j+1)(nx+2) + (i+1), each
+ * holding V unknowns of ncomp complex doubles, m = F..-F.  gpe_read_dat sets the level like
+ * gpe_set_state (level 0: run time t). */
+int gpe_write_dat(gpe_ctx *ctx, int level, const char *path);
+int gpe_read_dat(gpe_ctx *ctx, int level, const char *path, double t);
+/* Checkpoint pair of the reference (written :1690-1699, resumed :1228-1307): files
+ *   <dir>/<prefix><T>.dat        level 1, psi_{k-1} (the reference's odd array)
+ *   <dir>/<prefix>even_<T>.dat   level 0, psi_k
+ *   <dir>/<prefix><T>.clock      text: t0 and sweeps of the run clock (exact resume)
+ * T: the run time printed as the reference's toString (fixed, 18 decimals, utils.h:102-108),
+ * copied to tag (tag_cap bytes, may be NULL).  Needs a started context. */
+int gpe_save_checkpoint(gpe_ctx *ctx, const char *dir, const char *prefix, char *tag, int tag_cap);
+/* Resume from <dir>/<prefix><tag>.dat.  With the even file present both levels are loaded and
+ * the context steps on directly (*levels = 2, the reference skips its forward-Euler step);
+ * without it level 0 is loaded and the caller starts (gpe_start) as the reference does
+ * (*levels = 1).  The clock comes from the .clock file when present, else t = the tag's value. */
+int gpe_load_checkpoint(gpe_ctx *ctx, const char *dir, const char *prefix, const char *tag, int *levels);
+
+/* -------------------------------------------------------------------- VTK files ---- */
+/* Legacy binary VTK POLYDATA files laid out byte for byte as the reference writes them, so its
+ * viewers (ferro_skyrmion.py, bn_skyrmion.py, integrate_dens.py: arrays read by index) read
+ * them unchanged.  Points: every unknown of the one-block-padded grid in the order z, x, y, v,
+ * at p0 + s (i_padded * block edge + pos_v) as big-endian float32 (the padded index, as in the
+ * reference, which places the points one block edge above the unknowns' positions; halo
+ * points carry zeros).  Values 0 where the density is at most 1e-4 (utils.h:789).
+ *   spinor:    SCALARS r_m=F, i_m=F, ..., r_m=-F, i_m=-F: psi_m / sqrt(n)      (saveSpinor,
+ *              Spin1GpeBCC/utils.h:1079-1314, Spin2GpeBCC/utils.h:1071-1744); spin-2 basis
+ *              GPE_BASIS_Z, _Y, _X quantisation axis (Spin2GpeBCC/AliceRingRamps.h:6-10)
+ *   preimage:  spin-1: the spinor arrays on the interior points whose spin direction
+ *              psi^dag F psi / |psi^dag F psi| has |component axis| > threshold (reference: y,
+ *              0.9996; savePreImageSpinor, Spin1GpeBCC/utils.h:1351-1715)
+ *   volume:    spin-1: density, spinNorm, theta, VECTORS localAvgSpin, VECTORS u
+ *              (saveVolume, Spin1GpeBCC/utils.h:792-1077; localAvgSpin as the reference's
+ *              localAvgSpinAndDensity, Spin1GpeBCC.cu:235-271, u and theta as uvTheta,
+ *              :273-326); spin-2: SCALARS m=2..m=-2 densities (Spin2GpeBCC/utils.h:809-1069) */
+#define GPE_BASIS_Z 0
+#define GPE_BASIS_Y 1
+#define GPE_BASIS_X 2
+int gpe_write_vtk_spinor(gpe_ctx *ctx, int level, const char *path, int basis);
+int gpe_write_vtk_preimage(gpe_ctx *ctx, int level, const char *path, int axis, double threshold);
+int gpe_write_vtk_volume(gpe_ctx *ctx, int level, const char *path, int basis);
+
+/* The same formats for a state held on the host (ABI layout, complex128) on the grid of cfg,
+ * without a context or a device: post-processing of saved states, and the host side of the
+ * checkpoint pair.  gpe_host_read_dat fills psi (ncomp * nz * ny * nx * V complex values). */
+int gpe_host_write_dat(const gpe_config *cfg, const double *psi, const char *path);
+int gpe_host_read_dat(const gpe_config *cfg, const char *path, double *psi);
+int gpe_host_write_vtk_spinor(const gpe_config *cfg, const double *psi, const char *path, int basis);
+int gpe_host_write_vtk_preimage(const gpe_config *cfg, const double *psi, const char *path, int axis,
+                                double threshold);
+int gpe_host_write_vtk_volume(const gpe_config *cfg, const double *psi, const char *path, int basis);
 
 #ifdef __cplusplus
 }

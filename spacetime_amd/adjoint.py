@@ -13,7 +13,7 @@ from . import gpe, states
 
 class RampTrapProblem:
     def __init__(self, config, psi0, nsweeps, checkpoint_every, knot_duration, objective=gpe.OBJ_OVERLAP,
-                 target=None, bq=0.0):
+                 target=None, bq=0.0, fp64_trajectory=False):
         self.config = config
         self.psi0 = np.ascontiguousarray(psi0, dtype=np.complex128)
         self.nsweeps = int(nsweeps)
@@ -22,6 +22,7 @@ class RampTrapProblem:
         self.objective = objective
         self.target = None if target is None else np.ascontiguousarray(target, dtype=np.complex128)
         self.bq = float(bq)
+        self.fp64_trajectory = bool(fp64_trajectory)
         self.stepper = gpe.Stepper(config)
         self.last = None
 
@@ -38,7 +39,8 @@ class RampTrapProblem:
     def value_and_grad(self, knots, trap):
         """J, dJ/dknots (Bz values), dJ/dtrap by one forward + adjoint pass."""
         self._prepare(knots, trap)
-        res = self.stepper.adjoint(self.nsweeps, self.checkpoint_every, self.objective, self.target)
+        res = self.stepper.adjoint(self.nsweeps, self.checkpoint_every, self.objective, self.target,
+                                   fp64_trajectory=self.fp64_trajectory)
         self.last = res
         return res["J"], np.array(res["grad_ramp_b"][:, 2]), np.array(res["grad_trap"])
 

@@ -293,6 +293,31 @@ bool finish_geometry(Geometry &g)
     return true;
 }
 
+// Layout of a configuration's levels and its default traversal (one x per lane, tiles of 8 rows,
+// XCD remap), before finish_geometry.
+Geometry base_geometry(const gpe_config &cfg)
+{
+    Geometry g{};
+    g.nx = cfg.nx;
+    g.ny = cfg.ny;
+    g.nz = cfg.nz;
+    g.V = kLattices[cfg.lattice].V;
+    g.deg = kLattices[cfg.lattice].deg;
+    g.C = cfg.ncomp;
+    g.xpl = 1;
+    g.ty = cfg.ny < 8 ? cfg.ny : 8;
+    // x-lines start on 128-byte boundaries unless that pads a line by more than 1/20 (the padding
+    // is fetched and written in part with every line), then on 16 / 32 bytes (an even number of
+    // values, which keeps the x-pair accesses 16-byte aligned).  Spin-2 FP32, even against
+    // 128-byte lines: BCC 115^3 (11% padding) +8.5%, BCC 118^3 (8.5%) +5%, A15 76^3 (5.3%) +2.4%;
+    // lines padded by less ran 1-3% faster with the 128-byte start.
+    const long long align = cfg.precision == GPE_FP64 ? 8 : 16;
+    g.lx = ((long long)cfg.nx + align - 1) / align * align;
+    if (20 * g.lx > 21 * (long long)cfg.nx) g.lx = ((long long)cfg.nx + 1) / 2 * 2;
+    g.xcd_remap = 1;
+    return g;
+}
+
 SweepArgs sweep_args(gpe_ctx *c, int mode_coef_sweep, const void *cur, void *next, double coef)
 {
     SweepArgs a;
@@ -320,6 +345,25 @@ int check_level(gpe_ctx *c, int level)
 }
 
 }  // namespace
+
+namespace gpe {
+
+int ctx_view(gpe_ctx *c, CtxView *out)
+{
+    if (!c || !out) return GPE_EINVAL;
+    out->g = c->g;
+    out->fp64 = c->fp64;
+    out->ncomp = c->cfg.ncomp;
+    out->device = c->device;
+    out->stream = c->stream;
+    out->level[0] = c->level[0];
+    out->level[1] = c->level[1];
+    return GPE_OK;
+}
+
+int ctx_fail(gpe_ctx *c, int code, const char *what) { return fail(c, code, what); }
+
+}  // namespace gpe
 
 extern "C" {
 
@@ -354,23 +398,7 @@ int gpe_create(const gpe_config *cfg, gpe_ctx **out)
     c->elem_bytes = c->fp64 ? 16 : 8;
 
     Geometry &g = c->g;
-    g.nx = cfg->nx;
-    g.ny = cfg->ny;
-    g.nz = cfg->nz;
-    g.V = c->lat->V;
-    g.deg = c->lat->deg;
-    g.C = cfg->ncomp;
-    g.xpl = 1;
-    g.ty = cfg->ny < 8 ? cfg->ny : 8;
-    // x-lines start on 128-byte boundaries unless that pads a line by more than 1/20 (the padding
-    // is fetched and written in part with every line), then on 16 / 32 bytes (an even number of
-    // values, which keeps the x-pair accesses 16-byte aligned).  Spin-2 FP32, even against
-    // 128-byte lines: BCC 115^3 (11% padding) +8.5%, BCC 118^3 (8.5%) +5%, A15 76^3 (5.3%) +2.4%;
-    // lines padded by less ran 1-3% faster with the 128-byte start.
-    const long long align = c->fp64 ? 8 : 16;
-    g.lx = ((long long)cfg->nx + align - 1) / align * align;
-    if (20 * g.lx > 21 * (long long)cfg->nx) g.lx = ((long long)cfg->nx + 1) / 2 * 2;
-    g.xcd_remap = 1;
+    g = base_geometry(*cfg);
     c->gs = g;
     c->gs.xpl = default_xpl(c->fp64, g.deg, cfg->ncomp, cfg->nx);
     if (!finish_geometry(g) || !finish_geometry(c->gs)) {
@@ -835,6 +863,83 @@ int gpe_set_trap(gpe_ctx *c, const double trap[3])
     return GPE_OK;
 }
 
+int gpe_get_config(const gpe_ctx *c, gpe_config *out)
+{
+    if (!c || !out) return GPE_EINVAL;
+    *out = c->cfg;
+    return GPE_OK;
+}
+
+int gpe_set_time(gpe_ctx *c, double t0, long long sweeps)
+{
+    if (!c) return GPE_EINVAL;
+    if (sweeps < 0 || !std::isfinite(t0)) return fail(c, GPE_EINVAL, "bad clock");
+    c->t0 = t0;
+    c->sweeps = sweeps;
+    return GPE_OK;
+}
+
+int gpe_clock(const gpe_ctx *c, double *t0, long long *sweeps)
+{
+    if (!c) return GPE_EINVAL;
+    if (t0) *t0 = c->t0;
+    if (sweeps) *sweeps = c->sweeps;
+    return GPE_OK;
+}
+
+int gpe_project_spinor(gpe_ctx *c, int level, int phase, double param)
+{
+    if (!c) return GPE_EINVAL;
+    int rc = check_level(c, level);
+    if (rc) return rc;
+    const int C = c->cfg.ncomp;
+    const bool spin1 = C == 3 && (phase == GPE_PHASE_POLAR || phase == GPE_PHASE_FERRO || phase == GPE_PHASE_MIXED);
+    const bool spin2 = C == 5 && (phase == GPE_PHASE_FERRO || (phase >= GPE_PHASE_UN && phase <= GPE_PHASE_CYCLIC));
+    if (!spin1 && !spin2) return fail(c, GPE_EINVAL, "phase does not exist for this component count");
+    if (phase == GPE_PHASE_MIXED && !(param >= 0.0 && param <= 1.0)) return fail(c, GPE_EINVAL, "mix outside [0, 1]");
+    hipError_t e = launch_project(c->fp64, C, c->g, c->level[level], phase, param, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e == hipSuccess ? GPE_OK : hip_fail(c, e, "spinor projection");
+}
+
+int gpe_add_noise(gpe_ctx *c, int level, double amplitude, unsigned long long seed)
+{
+    if (!c) return GPE_EINVAL;
+    int rc = check_level(c, level);
+    if (rc) return rc;
+    if (!(amplitude >= 0.0)) return fail(c, GPE_EINVAL, "negative noise amplitude");
+    hipError_t e = launch_noise(c->fp64, c->cfg.ncomp, c->g, c->level[level], amplitude, seed, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e == hipSuccess ? GPE_OK : hip_fail(c, e, "noise");
+}
+
+int gpe_set_knot_ramp(gpe_ctx *c, int n, const double *knots, double duration, double bq)
+{
+    if (!c) return GPE_EINVAL;
+    if (n < 2 || !knots || !(duration > 0.0)) return fail(c, GPE_EINVAL, "need >= 2 knots and a positive duration");
+    std::vector<double> bval(3 * (size_t)n, 0.0), bdur(n), qval{bq}, qdur{1e300};
+    std::vector<int> btype(n), qtype{0};
+    for (int j = 0; j < n; ++j) {
+        bval[3 * j + 2] = knots[j];
+        bdur[j] = j == 0 ? 0.0 : duration / (n - 1);
+        btype[j] = j == 0 ? 0 : 1;
+    }
+    gpe_ramp r{};
+    r.nq = 1;
+    r.q_val = qval.data();
+    r.q_dur = qdur.data();
+    r.q_type = qtype.data();
+    r.nb = n;
+    r.b_val = bval.data();
+    r.b_dur = bdur.data();
+    r.b_type = btype.data();
+    r.tau = 1.0;
+    r.time_scale = 1.0;
+    r.scale_q = r.scale_b = 1.0;
+    r.qscale_q = r.qscale_b = 0.0;
+    return gpe_set_ramp(c, &r);
+}
+
 // Ramp sensitivities: the ramp value is linear in the segment values, so
 // d value(t) / d val[j] = value(t) evaluated with val = e_j.
 static double ramp_basis(const std::vector<double> &dur, const std::vector<int> &type, double tau, double t, int j)
@@ -844,6 +949,53 @@ static double ramp_basis(const std::vector<double> &dur, const std::vector<int> 
     double out;
     ramp_channel(e, dur, type, 1, tau, t, &out);
     return out;
+}
+
+// The adjoint of an FP32 context with FP64 psi and mu: the run (configuration, fields, ramp,
+// clock and level 0) is copied into a temporary FP64 context, the adjoint taken there, and the
+// final two levels copied back.
+static int adjoint_fp64_copy(gpe_ctx *c, const gpe_adjoint_spec *spec, gpe_adjoint_result *res)
+{
+    gpe_config cfg = c->cfg;
+    cfg.precision = GPE_FP64;
+    gpe_ctx *d = nullptr;
+    int rc = gpe_create(&cfg, &d);
+    if (rc) return fail(c, rc, "the FP64 copy of the run does not fit");
+    d->ramp = c->ramp;
+    d->bq = c->bq;
+    d->bqq = c->bqq;
+    for (int k = 0; k < 3; ++k) {
+        d->bb[k] = c->bb[k];
+        d->bbq[k] = c->bbq[k];
+    }
+    const size_t abi = (size_t)cfg.ncomp * c->g.nz * c->g.ny * c->g.nx * c->g.V * 2 * sizeof(double);
+    double *tmp = nullptr;
+    if (hipMalloc((void **)&tmp, abi) != hipSuccess) {
+        gpe_destroy(d);
+        return fail(c, GPE_ENOMEM, "staging for the FP64 copy does not fit");
+    }
+    const double t0 = c->t0;
+    rc = gpe_get_state_device(c, 0, tmp);
+    if (!rc) rc = gpe_set_state_device(d, 0, tmp, t0);
+    gpe_adjoint_spec s64 = *spec;
+    s64.fp64_trajectory = 0;
+    if (!rc) {
+        rc = gpe_adjoint(d, &s64, res);
+        if (rc) c->err = "FP64 copy: " + d->err;
+    }
+    if (!rc) rc = gpe_get_state_device(d, 0, tmp);
+    if (!rc) rc = gpe_set_state_device(c, 0, tmp, t0);
+    if (!rc) rc = gpe_get_state_device(d, 1, tmp);
+    if (!rc) rc = gpe_set_state_device(c, 1, tmp, t0);
+    if (!rc) {
+        c->sweeps = d->sweeps;
+        c->level1_valid = true;
+    } else {
+        c->level1_valid = false;
+    }
+    (void)hipFree(tmp);
+    gpe_destroy(d);
+    return rc;
 }
 
 int gpe_adjoint(gpe_ctx *c, const gpe_adjoint_spec *spec, gpe_adjoint_result *res)
@@ -860,6 +1012,7 @@ int gpe_adjoint(gpe_ctx *c, const gpe_adjoint_spec *spec, gpe_adjoint_result *re
     if (spec->objective < 0 || spec->objective > 2) return fail(c, GPE_EINVAL, "unknown objective");
     if (spec->objective == GPE_OBJ_OVERLAP && !spec->target) return fail(c, GPE_EINVAL, "overlap needs a target");
     if (c->sweeps != 0) return fail(c, GPE_ESTATE, "adjoint must start right after gpe_set_state");
+    if (spec->fp64_trajectory && !c->fp64) return adjoint_fp64_copy(c, spec, res);
 
     const size_t lbytes = (size_t)c->level_elems * c->elem_bytes;
     const int nck = (N + K - 1) / K;
@@ -887,7 +1040,9 @@ int gpe_adjoint(gpe_ctx *c, const gpe_adjoint_spec *spec, gpe_adjoint_result *re
     if (!rc) rc = alloc((void **)&d_part, sizeof(double) * ADJ_COUNT * (size_t)c->g.blocks);
     // per-sweep sensitivity partials of up to Q consecutive adjoint sweeps, summed in one launch
     // (one two-stage finalize per sweep cost ~15% of the backward pass in launches)
-    const size_t sweep_part = (size_t)ADJ_COUNT * c->g.blocks;
+    // the adjoint sweeps run on the forward sweep's traversal (x pairs in FP32)
+    const int adj_blocks = c->gs.blocks;
+    const size_t sweep_part = (size_t)ADJ_COUNT * adj_blocks;
     const int Q = (int)std::max<size_t>(1, std::min<size_t>((size_t)K, ((size_t)256 << 20) / (sweep_part * 8)));
     double *d_qpart = nullptr;
     if (!rc) rc = alloc((void **)&d_qpart, sizeof(double) * sweep_part * Q);
@@ -1032,7 +1187,7 @@ int gpe_adjoint(gpe_ctx *c, const gpe_adjoint_spec *spec, gpe_adjoint_result *re
         int qhi = khi, nq = 0;  // pending sweeps qhi, qhi - 1, ... in slots 0, 1, ...
         for (int k = khi; k >= klo; --k) {
             AdjArgs aa;
-            aa.g = c->g;
+            aa.g = c->gs;
             aa.tb = tables(c);
             aa.ph = c->ph;
             aa.psi = seg[k - klo];
@@ -1046,7 +1201,7 @@ int gpe_adjoint(gpe_ctx *c, const gpe_adjoint_spec *spec, gpe_adjoint_result *re
             aa.partials = d_qpart + sweep_part * nq;
             ADJ_CHECK(launch_adjoint(c->fp64, c->cfg.ncomp, aa, c->stream), "adjoint sweep");
             if (++nq == Q || k == klo) {
-                ADJ_CHECK(launch_sum_tables(d_qpart, nq, c->g.blocks, ADJ_COUNT, d_steps + (size_t)ADJ_COUNT * qhi,
+                ADJ_CHECK(launch_sum_tables(d_qpart, nq, adj_blocks, ADJ_COUNT, d_steps + (size_t)ADJ_COUNT * qhi,
                                             -(long long)ADJ_COUNT, c->stream),
                           "adjoint sums");
                 qhi = k - 1;
@@ -1379,6 +1534,46 @@ int gpe_traversal(const gpe_ctx *c, int *tile_rows, int *xcd_remap, int *x_per_l
     if (tile_rows) *tile_rows = c->gs.ty;
     if (xcd_remap) *xcd_remap = c->gs.xcd_remap;
     if (x_per_lane) *x_per_lane = c->gs.xpl;
+    return GPE_OK;
+}
+
+int gpe_plan_traversal(const gpe_config *cfg, int tile_rows, int xcd_remap, int x_per_lane, gpe_traversal_plan *out)
+{
+    if (!cfg || !out) return GPE_EINVAL;
+    if (cfg->lattice < 0 || cfg->lattice > 5 || (cfg->ncomp != 1 && cfg->ncomp != 3 && cfg->ncomp != 5)) return GPE_EINVAL;
+    if (cfg->precision != GPE_FP32 && cfg->precision != GPE_FP64) return GPE_EINVAL;
+    if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1) return GPE_EINVAL;
+    if (tile_rows < 1 || tile_rows > 64 || (xcd_remap != 0 && xcd_remap != 1) || (x_per_lane != 1 && x_per_lane != 2))
+        return GPE_EINVAL;
+    Geometry g = base_geometry(*cfg);
+    if (x_per_lane == 2 && !pair_lanes_ok(cfg->precision == GPE_FP64, g.deg, cfg->ncomp)) return GPE_EINVAL;
+    g.ty = tile_rows;
+    g.xcd_remap = xcd_remap;
+    g.xpl = x_per_lane;
+    if (!finish_geometry(g)) return GPE_EINVAL;
+    out->nx = g.nx;
+    out->ny = g.ny;
+    out->nz = g.nz;
+    out->V = g.V;
+    out->C = g.C;
+    out->x_per_lane = g.xpl;
+    out->rows = g.rows;
+    out->pairs = g.pairs;
+    out->nseg = g.nseg;
+    out->tile_rows = g.ty;
+    out->row_groups = g.tyr;
+    out->ntile = g.ntile;
+    out->xcd_remap = g.xcd_remap;
+    out->lx = g.lx;
+    out->waves = g.waves;
+    out->blocks = g.blocks;
+    const FastDiv *d[5] = {&g.f_nseg, &g.f_V, &g.f_ty, &g.f_nz, &g.f_pairs};
+    unsigned *o[5] = {out->div_nseg, out->div_V, out->div_rows, out->div_nz, out->div_pairs};
+    for (int k = 0; k < 5; ++k) {
+        o[k][0] = d[k]->d;
+        o[k][1] = d[k]->m;
+        o[k][2] = d[k]->l;
+    }
     return GPE_OK;
 }
 

@@ -6,6 +6,8 @@
 
 #include <cstdint>
 
+#include "gpe.h"
+
 namespace gpe {
 
 // Division of a 32-bit unsigned n by an invariant d >= 1 as a multiply-high and shifts
@@ -248,5 +250,19 @@ hipError_t launch_unpack(int fp64, const Geometry &g, int ncomp, const void *dev
 // dst = phase * src (complex), for the staggered start
 hipError_t launch_phase_copy(int fp64, const void *src, void *dst, long long nelem, double pre, double pim,
                              hipStream_t s);
+// What other source files of the library see of a context (gpe_host.cpp owns the struct).
+struct CtxView {
+    Geometry g;          // one x per lane: the layout of a level
+    int fp64, ncomp, device;
+    hipStream_t stream;
+    void *level[2];
+};
+int ctx_view(gpe_ctx *c, CtxView *out);
+int ctx_fail(gpe_ctx *c, int code, const char *what);
+
+// spinor phase projection (GPE_PHASE_*) and spinor noise of a level, in place
+hipError_t launch_project(int fp64, int ncomp, const Geometry &g, void *psi, int phase, double param, hipStream_t s);
+hipError_t launch_noise(int fp64, int ncomp, const Geometry &g, void *psi, double amp, unsigned long long seed,
+                        hipStream_t s);
 
 }  // namespace gpe

@@ -861,6 +861,140 @@ __global__ __launch_bounds__(256) void k_fill_tf(Geometry g, Tables tb, Physics 
     }
 }
 
+// Unknown (x, v, y, z) of flat index i, x fastest: consecutive lanes take consecutive x of one
+// line of the device layout.
+struct Unknown {
+    int x, v, y, z;
+};
+
+__device__ __forceinline__ Unknown unknown_of(const Geometry &g, long long i)
+{
+    Unknown u;
+    u.x = (int)(i % g.nx);
+    i /= g.nx;
+    u.v = (int)(i % g.V);
+    i /= g.V;
+    u.y = (int)(i % g.ny);
+    u.z = (int)(i / g.ny);
+    return u;
+}
+
+// Spinor phase projection keeping the density n of every unknown (phases of gpe.h
+// GPE_PHASE_*): spin-1 polar (0, sqrt n, 0), ferromagnetic (sqrt n, 0, 0), mixed
+// (sqrt(n mix), sqrt(n (1 - mix)), 0) (Spin1GpeBCC.cu:463-551); spin-2 UN, horizontal and vertical
+// BN, cyclic with the relative phase `param` (Spin2GpeBCC.cu:417-558), ferromagnetic.  The
+// density is summed over m = F..-F and the amplitudes formed as in the reference.
+template <typename R, int C>
+__global__ __launch_bounds__(256) void k_project(Geometry g, vec2_t<R> *psi, int phase, double param)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long long)g.nz * g.ny * g.V * g.nx) return;
+    const Unknown u = unknown_of(g, i);
+    double n = 0.0;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const vec2_t<R> o = psi[line_index(g, c, u.z, u.y, u.v) + u.x];
+        const double re = o.x, im = o.y;
+        n += re * re + im * im;
+    }
+    double outr[C], outi[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) outr[c] = outi[c] = 0.0;
+    const double cp = cos(param), sp = sin(param);
+    if constexpr (C == 3) {
+        if (phase == 0) {
+            outr[1] = sqrt(n);
+        } else if (phase == 1) {
+            outr[0] = sqrt(n);
+        } else {
+            outr[0] = sqrt(n * param);
+            outr[1] = sqrt(n * (1.0 - param));
+        }
+    } else {
+        if (phase == 1) {
+            outr[0] = sqrt(n);
+        } else if (phase == 3) {
+            outr[2] = sqrt(n);
+        } else if (phase == 4 || phase == 5) {
+            const double a = sqrt(n / 2);
+            const int lo = phase == 4 ? 0 : 1;
+            outr[lo] = a;
+            outr[4 - lo] = cp * a;
+            outi[4 - lo] = sp * a;
+        } else {
+            const double a2 = sqrt(n * 1 / 3), a1 = sqrt(n * 2 / 3);
+            outr[0] = a2;
+            outr[3] = cp * a1;
+            outi[3] = sp * a1;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        vec2_t<R> o;
+        o.x = (R)outr[c];
+        o.y = (R)outi[c];
+        psi[line_index(g, c, u.z, u.y, u.v) + u.x] = o;
+    }
+}
+
+// psi_m += amp sqrt(n) eps_m, eps_m uniform in the unit disk from the counter hash of (seed,
+// unknown in ABI order, 2m / 2m + 1) -- the variates of k_fill_tf -- n the density before.
+template <typename R, int C>
+__global__ __launch_bounds__(256) void k_noise(Geometry g, vec2_t<R> *psi, double amp, unsigned long long seed)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long long)g.nz * g.ny * g.V * g.nx) return;
+    const Unknown u = unknown_of(g, i);
+    const unsigned long long node = (((unsigned long long)u.z * g.ny + u.y) * g.nx + u.x) * g.V + u.v;
+    double pr[C], pi[C], n = 0.0;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const vec2_t<R> o = psi[line_index(g, c, u.z, u.y, u.v) + u.x];
+        pr[c] = o.x;
+        pi[c] = o.y;
+        n += pr[c] * pr[c] + pi[c] * pi[c];
+    }
+    const double a = amp * sqrt(n);
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const double rad = sqrt(hash_uniform(seed, node, 2 * c));
+        const double ang = 6.283185307179586 * hash_uniform(seed, node, 2 * c + 1);
+        vec2_t<R> o;
+        o.x = (R)(pr[c] + a * rad * cos(ang));
+        o.y = (R)(pi[c] + a * rad * sin(ang));
+        psi[line_index(g, c, u.z, u.y, u.v) + u.x] = o;
+    }
+}
+
+hipError_t launch_project(int fp64, int ncomp, const Geometry &g, void *psi, int phase, double param, hipStream_t s)
+{
+    const unsigned grid = (unsigned)(((long long)g.nz * g.ny * g.V * g.nx + 255) / 256);
+    if (ncomp == 3) {
+        if (fp64) hipLaunchKernelGGL((k_project<double, 3>), dim3(grid), dim3(256), 0, s, g, (double2 *)psi, phase, param);
+        else hipLaunchKernelGGL((k_project<float, 3>), dim3(grid), dim3(256), 0, s, g, (float2 *)psi, phase, param);
+    } else if (ncomp == 5) {
+        if (fp64) hipLaunchKernelGGL((k_project<double, 5>), dim3(grid), dim3(256), 0, s, g, (double2 *)psi, phase, param);
+        else hipLaunchKernelGGL((k_project<float, 5>), dim3(grid), dim3(256), 0, s, g, (float2 *)psi, phase, param);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_noise(int fp64, int ncomp, const Geometry &g, void *psi, double amp, unsigned long long seed,
+                        hipStream_t s)
+{
+    const unsigned grid = (unsigned)(((long long)g.nz * g.ny * g.V * g.nx + 255) / 256);
+#define NZ(R, C) hipLaunchKernelGGL((k_noise<R, C>), dim3(grid), dim3(256), 0, s, g, (vec2_t<R> *)psi, amp, seed)
+    if (fp64) {
+        if (ncomp == 1) NZ(double, 1); else if (ncomp == 3) NZ(double, 3); else NZ(double, 5);
+    } else {
+        if (ncomp == 1) NZ(float, 1); else if (ncomp == 3) NZ(float, 3); else NZ(float, 5);
+    }
+#undef NZ
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_tf(int fp64, const Geometry &g, const Tables &tb, const Physics &ph, int ncomp, const FillArgs &f,
                           void *psi, hipStream_t s)
 {

@@ -25,6 +25,8 @@ LEAPFROG3, STAGGERED = 0, 1
 START_EULER, START_PHASE, START_COPY = 0, 1, 2
 OBJ_OVERLAP, OBJ_ENERGY, OBJ_MAGNETIZATION_Z = 0, 1, 2
 FIELD_DENSITY, FIELD_SPIN, FIELD_SPIN_NORM, FIELD_UV_THETA = 0, 1, 2, 3
+PHASES = {"polar": 0, "ferro": 1, "mixed": 2, "un": 3, "bn_hori": 4, "bn_vert": 5, "cyclic": 6}
+BASIS_Z, BASIS_Y, BASIS_X = 0, 1, 2
 ERRORS = {-22: "EINVAL", -12: "ENOMEM", -5: "EDEVICE", -71: "ESTATE", -34: "ERANGE"}
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -63,7 +65,7 @@ class _Obs(ctypes.Structure):
 
 class _AdjSpec(ctypes.Structure):
     _fields_ = [("nsweeps", ctypes.c_int), ("checkpoint_every", ctypes.c_int), ("objective", ctypes.c_int),
-                ("target", ctypes.c_void_p)]
+                ("target", ctypes.c_void_p), ("fp64_trajectory", ctypes.c_int)]
 
 
 class _AdjRes(ctypes.Structure):
@@ -128,6 +130,26 @@ def load_library(path=LIB_PATH):
     lib.gpe_local_fields.argtypes = [vp, ctypes.c_int, ctypes.c_int, _dp]
     lib.gpe_fill_thomas_fermi.argtypes = [vp, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_double,
                                           ctypes.c_ulonglong, ctypes.c_double]
+    lib.gpe_get_config.argtypes = [vp, ctypes.POINTER(_Config)]
+    lib.gpe_set_time.argtypes = [vp, ctypes.c_double, ctypes.c_longlong]
+    lib.gpe_clock.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_longlong)]
+    lib.gpe_project_spinor.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+    lib.gpe_add_noise.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_ulonglong]
+    lib.gpe_set_knot_ramp.argtypes = [vp, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double]
+    lib.gpe_write_dat.argtypes = [vp, ctypes.c_int, ctypes.c_char_p]
+    lib.gpe_read_dat.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_double]
+    lib.gpe_save_checkpoint.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
+    lib.gpe_load_checkpoint.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                        ctypes.POINTER(ctypes.c_int)]
+    lib.gpe_write_vtk_spinor.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    lib.gpe_write_vtk_preimage.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.c_double]
+    lib.gpe_write_vtk_volume.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    cp = ctypes.POINTER(_Config)
+    lib.gpe_host_write_dat.argtypes = [cp, vp, ctypes.c_char_p]
+    lib.gpe_host_read_dat.argtypes = [cp, ctypes.c_char_p, vp]
+    lib.gpe_host_write_vtk_spinor.argtypes = [cp, vp, ctypes.c_char_p, ctypes.c_int]
+    lib.gpe_host_write_vtk_preimage.argtypes = [cp, vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_double]
+    lib.gpe_host_write_vtk_volume.argtypes = [cp, vp, ctypes.c_char_p, ctypes.c_int]
     _LIB = lib
     return lib
 
@@ -304,12 +326,14 @@ class Stepper:
         w = (ctypes.c_double * 3)(*trap)
         self._check(self.lib.gpe_set_trap(self._h, w), "gpe_set_trap")
 
-    def adjoint(self, nsweeps, checkpoint_every, objective=OBJ_OVERLAP, target=None):
+    def adjoint(self, nsweeps, checkpoint_every, objective=OBJ_OVERLAP, target=None, fp64_trajectory=False):
         """Objective and its gradients by the discrete adjoint (gpe_adjoint).  Returns a dict
         with J, grad_trap (3), grad_static_bq, grad_static_bb (3), grad_ramp_q (nq),
-        grad_ramp_b (nb, 3), timings and memory."""
+        grad_ramp_b (nb, 3), timings and memory.  fp64_trajectory: FP64 psi and mu on an FP32
+        context (gradients that are small residues of cancelling per-sweep terms need it)."""
         spec = _AdjSpec()
         spec.nsweeps, spec.checkpoint_every, spec.objective = int(nsweeps), int(checkpoint_every), objective
+        spec.fp64_trajectory = 1 if fp64_trajectory else 0
         tgt = None
         if target is not None:
             tgt = np.ascontiguousarray(target, dtype=np.complex128)
@@ -329,6 +353,59 @@ class Stepper:
                     checkpoint_bytes=res.checkpoint_bytes, segment_bytes=res.segment_bytes)
 
     # -- time stepping -------------------------------------------------------------------------
+    def clock(self):
+        """(t0, sweeps) of the run clock (gpe_clock)."""
+        t0, n = ctypes.c_double(), ctypes.c_longlong()
+        self._check(self.lib.gpe_clock(self._h, ctypes.byref(t0), ctypes.byref(n)), "gpe_clock")
+        return t0.value, n.value
+
+    def set_time(self, t0, sweeps=0):
+        self._check(self.lib.gpe_set_time(self._h, float(t0), int(sweeps)), "gpe_set_time")
+
+    def project(self, phase, param=0.0, level=0):
+        """Spinor phase projection on the device (gpe_project_spinor): 'polar', 'ferro',
+        'mixed' (param = ferro fraction), 'un', 'bn_hori', 'bn_vert', 'cyclic' (param = the
+        relative phase)."""
+        self._check(self.lib.gpe_project_spinor(self._h, level, PHASES[phase], float(param)), "gpe_project_spinor")
+
+    def add_noise(self, amplitude, seed, level=0):
+        self._check(self.lib.gpe_add_noise(self._h, level, float(amplitude), int(seed)), "gpe_add_noise")
+
+    def set_knot_ramp(self, knots, duration, bq=0.0):
+        k = np.ascontiguousarray(knots, dtype=np.float64)
+        self._check(self.lib.gpe_set_knot_ramp(self._h, len(k), k.ctypes.data_as(_dp), float(duration), float(bq)),
+                    "gpe_set_knot_ramp")
+
+    def write_dat(self, path, level=0):
+        self._check(self.lib.gpe_write_dat(self._h, level, os.fsencode(path)), "gpe_write_dat")
+
+    def read_dat(self, path, level=0, t=0.0):
+        self._check(self.lib.gpe_read_dat(self._h, level, os.fsencode(path), float(t)), "gpe_read_dat")
+
+    def save_checkpoint(self, directory, prefix=""):
+        """Write the reference's checkpoint pair; returns the time tag T of the file names."""
+        tag = ctypes.create_string_buffer(512)
+        self._check(self.lib.gpe_save_checkpoint(self._h, os.fsencode(directory), prefix.encode(), tag, 512),
+                    "gpe_save_checkpoint")
+        return tag.value.decode()
+
+    def load_checkpoint(self, directory, tag, prefix=""):
+        """Resume from a checkpoint; returns the number of levels loaded (2: step on, 1: start)."""
+        n = ctypes.c_int()
+        self._check(self.lib.gpe_load_checkpoint(self._h, os.fsencode(directory), prefix.encode(), tag.encode(),
+                                                 ctypes.byref(n)), "gpe_load_checkpoint")
+        return n.value
+
+    def write_vtk_spinor(self, path, level=0, basis=BASIS_Z):
+        self._check(self.lib.gpe_write_vtk_spinor(self._h, level, os.fsencode(path), basis), "gpe_write_vtk_spinor")
+
+    def write_vtk_preimage(self, path, level=0, axis=1, threshold=0.9996):
+        self._check(self.lib.gpe_write_vtk_preimage(self._h, level, os.fsencode(path), axis, float(threshold)),
+                    "gpe_write_vtk_preimage")
+
+    def write_vtk_volume(self, path, level=0, basis=BASIS_Z):
+        self._check(self.lib.gpe_write_vtk_volume(self._h, level, os.fsencode(path), basis), "gpe_write_vtk_volume")
+
     def start(self, mode=START_EULER, mu=0.0):
         self._check(self.lib.gpe_start(self._h, mode, mu), "gpe_start")
 

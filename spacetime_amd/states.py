@@ -12,6 +12,7 @@
   (Spin1GpeBCC.cu:463-551) and UN / BN / cyclic (Spin2GpeBCC.cu:417-558).
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -202,26 +203,31 @@ def knot_ramp(knots, duration):
     return segs
 
 
-def read_reference_dat(path, dims, ncomp, V=12):
-    """Load a state file of the reference solvers into the ABI layout (ncomp, nz, ny, nx, V).
+def read_reference_dat(path, cfg):
+    """Load a state file of the reference solvers into the ABI layout cfg.shape.
 
     The files are raw dumps of the padded array of blocks written with ofstream::write
     (Spin1GpeBCC.cu:1464-1467,1690-1699; read back at :1228-1237): (nz+2)(ny+2)(nx+2) blocks,
     block (i, j, k) at index (k+1)(ny+2)(nx+2) + (j+1)(nx+2) + (i+1), each block holding V
-    unknowns of ncomp complex doubles ordered m = F..-F (Complex3Vec s1, s0, s_1;
-    Complex5Vec s2..s_2; the scalar solvers' double2)."""
-    nx, ny, nz = dims
-    count = (nx + 2) * (ny + 2) * (nz + 2) * V * ncomp
-    raw = np.fromfile(path, dtype=np.complex128)
-    if raw.size < count:
-        raise ValueError("%s holds %d complex values, expected %d" % (path, raw.size, count))
-    blocks = raw[:count].reshape(nz + 2, ny + 2, nx + 2, V, ncomp)
-    return np.ascontiguousarray(blocks[1:-1, 1:-1, 1:-1].transpose(4, 0, 1, 2, 3))
+    unknowns of ncomp complex doubles ordered m = F..-F.  Read by libgpe's host code
+    (gpe_host_read_dat)."""
+    from . import gpe
+    lib = gpe.load_library()
+    out = np.zeros(cfg.shape, dtype=np.complex128)
+    rc = lib.gpe_host_read_dat(cfg.to_c(), os.fsencode(str(path)), out.ctypes.data)
+    if rc != 0:
+        raise ValueError("%s: not a state file of this grid (%d)" % (path, rc))
+    return out
 
 
-def write_reference_dat(path, psi):
-    """Write an ABI-layout state as the reference's padded block file (zero halo)."""
-    ncomp, nz, ny, nx, V = psi.shape
-    blocks = np.zeros((nz + 2, ny + 2, nx + 2, V, ncomp), dtype=np.complex128)
-    blocks[1:-1, 1:-1, 1:-1] = psi.transpose(1, 2, 3, 4, 0)
-    blocks.tofile(path)
+def write_reference_dat(path, cfg, psi):
+    """Write an ABI-layout state as the reference's padded block file (zero halo), by
+    libgpe's host code (gpe_host_write_dat)."""
+    from . import gpe
+    lib = gpe.load_library()
+    psi = np.ascontiguousarray(psi, dtype=np.complex128)
+    if psi.shape != cfg.shape:
+        raise ValueError("state shape %s, expected %s" % (psi.shape, cfg.shape))
+    rc = lib.gpe_host_write_dat(cfg.to_c(), psi.ctypes.data, os.fsencode(str(path)))
+    if rc != 0:
+        raise OSError("cannot write %s (%d)" % (path, rc))

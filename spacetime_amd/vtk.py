@@ -1,70 +1,92 @@
-"""Legacy binary VTK volume files laid out as the reference's `saveVolume` writes them
-(Spin1GpeBCC/utils.h:792-1080), so its Python viewers (ferro_skyrmion.py, integrate_dens.py:
-`tvtk.PolyDataReader`, fields looked up by name) read them unchanged:
+"""Legacy binary VTK files in the reference's layouts, written by libgpe's host code.
 
-    # vtk DataFile Version 3.0 / comment / BINARY / DATASET POLYDATA
-    POINTS n float                      one point per unknown, big-endian float32 x, y, z
-    POINT_DATA n
-    SCALARS density float 1             |psi|^2 summed over the components
-    SCALARS spinNorm float 1            |<F>|              (spinor states)
-    SCALARS theta float 1               spin-1 phase theta  (spin-1 states)
-    VECTORS localAvgSpin float          <F> = psi^dag F psi / |psi|^2 (spinor states)
+The reference's viewers read these files with ``tvtk.PolyDataReader`` and take the point
+arrays **by index** (``point_data.get_array(i)``), so the writers reproduce the reference's
+files byte for byte where its arithmetic is exact:
 
-Every field but the density is 0 where the density is at most DENSITY_THRESHOLD (utils.h:788,
-903-935).  The points follow this package's unknown order (z, y, x, v); the reference walks
-(z, x, y, v) over its halo-padded grid (utils.h:806-827), whose halo points carry zeros.
+* spinor (``saveSpinor``, Spin1GpeBCC/utils.h:1079-1314, Spin2GpeBCC/utils.h:1071-1744): arrays
+  ``r_m=F, i_m=F, ..., r_m=-F, i_m=-F`` = Re / Im psi_m / sqrt(n) -- what ferro_skyrmion.py
+  (:105-110), bn_skyrmion.py (:234-243) and integrate_dens.py (:198-202) read as arrays 0..2C-1;
+* pre-image (``savePreImageSpinor``, Spin1GpeBCC/utils.h:1351-1715): the spinor arrays on the
+  points whose spin direction has a y component above 0.9996 (ferro_skyrmion.py:112-123);
+* volume (``saveVolume``): spin-1 density, spinNorm, theta, localAvgSpin, u
+  (Spin1GpeBCC/utils.h:792-1077); spin-2 densities m=2..-2 (Spin2GpeBCC/utils.h:809-1069).
+
+Points: every unknown of the one-block-padded grid in the order z, x, y, v (the halo points
+carry zeros) at p0 + s (i_padded * block edge + pos_v), as the reference writes them.  Values
+are zero where the density is at most DENSITY_THRESHOLD.  Writers take a Stepper (a level on
+the device) or a Config and a host state in the ABI layout.
 """
+import os
+
 import numpy as np
 
-DENSITY_THRESHOLD = 1e-4  # utils.h:788
+from . import gpe
 
-_HEADER = b"# vtk DataFile Version 3.0\nComment if needed\nBINARY\n"
-
-
-def write_volume(path, positions, density, spin=None, theta=None):
-    """Write one volume file.  positions (..., 3); density (...); spin (..., 3) or None;
-    theta (...) or None; all over the same unknowns."""
-    pos = np.asarray(positions, dtype=np.float64).reshape(-1, 3)
-    n = pos.shape[0]
-    dens = np.asarray(density, dtype=np.float64).reshape(-1)
-    if dens.shape[0] != n:
-        raise ValueError("density has %d values for %d points" % (dens.shape[0], n))
-    live = dens > DENSITY_THRESHOLD
-
-    def be(a):
-        return np.ascontiguousarray(a, dtype=">f4").tobytes()
-
-    with open(path, "wb") as f:
-        f.write(_HEADER)
-        f.write(b"DATASET POLYDATA\nPOINTS %d float\n" % n)
-        f.write(be(pos))
-        f.write(b"\nPOINT_DATA %d\nSCALARS density float 1\nLOOKUP_TABLE default\n" % n)
-        f.write(be(dens))
-        if spin is not None:
-            s = np.asarray(spin, dtype=np.float64).reshape(-1, 3)
-            if s.shape[0] != n:
-                raise ValueError("spin has %d vectors for %d points" % (s.shape[0], n))
-            s = np.where(live[:, None], s, 0.0)
-            f.write(b"\nSCALARS spinNorm float 1\nLOOKUP_TABLE default\n")
-            f.write(be(np.sqrt(np.sum(s * s, axis=1))))
-        if theta is not None:
-            th = np.asarray(theta, dtype=np.float64).reshape(-1)
-            if th.shape[0] != n:
-                raise ValueError("theta has %d values for %d points" % (th.shape[0], n))
-            f.write(b"\nSCALARS theta float 1\nLOOKUP_TABLE default\n")
-            f.write(be(np.where(live, th, 0.0)))
-        if spin is not None:
-            f.write(b"\nVECTORS localAvgSpin float\n")
-            f.write(be(s))
-        f.write(b"\n")
+DENSITY_THRESHOLD = 1e-4  # Spin1GpeBCC/utils.h:789
 
 
-def read_volume(path):
-    """Read a file written by write_volume (or the reference's saveVolume): dict with
-    "points" (n, 3) and each named field ((n,) scalars, (n, 3) vectors), as float32."""
+def _host(name):
+    lib = gpe.load_library()
+    f = getattr(lib, name)
+    return lib, f
+
+
+def _state(cfg, psi):
+    psi = np.ascontiguousarray(psi, dtype=np.complex128)
+    if psi.shape != cfg.shape:
+        raise ValueError("state shape %s, expected %s" % (psi.shape, cfg.shape))
+    return psi
+
+
+def _call(rc, what):
+    if rc != 0:
+        raise gpe.GpeError("%s failed: %s" % (what, gpe.ERRORS.get(rc, rc)))
+
+
+def write_spinor(path, cfg, psi, basis=gpe.BASIS_Z):
+    """saveSpinor layout for a host state (gpe_host_write_vtk_spinor)."""
+    psi = _state(cfg, psi)
+    lib, f = _host("gpe_host_write_vtk_spinor")
+    c = cfg.to_c()
+    _call(f(c, psi.ctypes.data, os.fsencode(str(path)), basis), "gpe_host_write_vtk_spinor")
+
+
+def write_preimage(path, cfg, psi, axis=1, threshold=0.9996):
+    """savePreImageSpinor layout for a host spin-1 state (gpe_host_write_vtk_preimage)."""
+    psi = _state(cfg, psi)
+    lib, f = _host("gpe_host_write_vtk_preimage")
+    _call(f(cfg.to_c(), psi.ctypes.data, os.fsencode(str(path)), axis, float(threshold)),
+          "gpe_host_write_vtk_preimage")
+
+
+def write_volume(path, cfg, psi, basis=gpe.BASIS_Z):
+    """saveVolume layout for a host state (gpe_host_write_vtk_volume)."""
+    psi = _state(cfg, psi)
+    lib, f = _host("gpe_host_write_vtk_volume")
+    _call(f(cfg.to_c(), psi.ctypes.data, os.fsencode(str(path)), basis), "gpe_host_write_vtk_volume")
+
+
+def save_spinor(stepper, path, level=0, basis=gpe.BASIS_Z):
+    stepper.write_vtk_spinor(str(path), level, basis)
+
+
+def save_preimage(stepper, path, level=0, axis=1, threshold=0.9996):
+    stepper.write_vtk_preimage(str(path), level, axis, threshold)
+
+
+def save_volume(stepper, path, level=0, basis=gpe.BASIS_Z):
+    stepper.write_vtk_volume(str(path), level, basis)
+
+
+def read(path):
+    """Read a legacy binary POLYDATA file: (points (n, 3), [(name, array), ...]) with the point
+    arrays in file order -- the indices under which tvtk's point_data.get_array(i) returns
+    them -- scalars (n,), vectors (n, 3), as float32."""
     with open(path, "rb") as f:
         data = f.read()
-    out = {}
+    arrays = []
+    points = None
     pos = 0
 
     def line():
@@ -87,22 +109,19 @@ def read_volume(path):
             continue
         if words[0] == "POINTS":
             n = int(words[1])
-            out["points"] = floats(3 * n).reshape(n, 3)
+            points = floats(3 * n).reshape(n, 3)
         elif words[0] == "SCALARS":
             if line().split()[0] != "LOOKUP_TABLE":
                 raise ValueError("SCALARS without LOOKUP_TABLE")
-            out[words[1]] = floats(n)
+            arrays.append((words[1], floats(n)))
         elif words[0] == "VECTORS":
-            out[words[1]] = floats(3 * n).reshape(n, 3)
+            arrays.append((words[1], floats(3 * n).reshape(n, 3)))
+    return points, arrays
+
+
+def read_volume(path):
+    """Named access to a file's arrays: dict of "points" and each array by name."""
+    points, arrays = read(path)
+    out = {"points": points}
+    out.update(dict(arrays))
     return out
-
-
-def save_volume(stepper, path, level=0):
-    """Write the volume file of one time level of a Stepper: density, and for spinor states
-    <F> with its norm, and for spin-1 the phase theta (field maps from gpe_local_fields)."""
-    from . import gpe, states
-    cfg = stepper.config
-    dens = stepper.local_fields(gpe.FIELD_DENSITY, level)[..., 0]
-    spin = stepper.local_fields(gpe.FIELD_SPIN, level) if cfg.ncomp > 1 else None
-    theta = stepper.local_fields(gpe.FIELD_UV_THETA, level)[..., 6] if cfg.ncomp == 3 else None
-    write_volume(path, states.positions(cfg), dens, spin, theta)

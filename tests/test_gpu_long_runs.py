@@ -35,20 +35,19 @@ def _ramp(duration):
 
 
 def _ground_state(cfg, itp_steps):
-    """Thomas-Fermi start, FP64 imaginary time on the device, polar projection, 0.5% noise."""
-    from spacetime_amd import lattice, states
-    tf, mu = states.thomas_fermi(cfg, cfg.c0)
-    psi = np.zeros(cfg.shape, dtype=np.complex128)
-    psi[1] = tf
-    del tf
+    """On the device, FP64: polar Thomas-Fermi fill, imaginary time, polar projection, 0.5%
+    spinor noise, unit norm."""
+    from spacetime_amd import lattice
+    lam = cfg.trap[0] * cfg.trap[1] * cfg.trap[2]
+    mu = 0.5 * (15.0 * cfg.c0 * lam / (4.0 * np.pi)) ** 0.4  # unit-norm Thomas-Fermi chemical potential
     dt_itp = min(2e-4, 1.0 / (lattice.get("bcc").kinetic_bound(cfg.block_scale) + mu))
     with gpe.Stepper(dataclasses.replace(cfg, precision=gpe.FP64)) as st:
-        st.set_state(psi)
+        st.fill_thomas_fermi(cfg.c0, mu, np.array([0, 1, 0], dtype=complex))
         st.itp(itp_steps, dt_itp)
-        psi = st.get_state(0)
-    psi = states.spinor_noise(states.project(psi, "polar"), 0.005, 54363)
-    psi *= np.sqrt(1.0 / states.norm(cfg, psi))
-    return psi
+        st.project("polar")
+        st.add_noise(0.005, 54363)
+        st.scale(1.0 / np.sqrt(st.observables(0).norm))
+        return st.get_state(0)
 
 
 def test_fp64_matches_oracle_over_1e4_steps(oracle):

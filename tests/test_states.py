@@ -92,11 +92,12 @@ def test_imprints_noise_and_projections():
 
 
 def test_reference_dat_roundtrip_and_layout(tmp_path):
+    from spacetime_amd import gpe
     rng = np.random.default_rng(9)
-    dims = (3, 4, 2)
+    cfg = gpe.Config(lattice="bcc", ncomp=3, dims=(3, 4, 2))
     psi = rng.normal(size=(3, 2, 4, 3, 12)) + 1j * rng.normal(size=(3, 2, 4, 3, 12))
     path = tmp_path / "state.dat"
-    states.write_reference_dat(str(path), psi)
+    states.write_reference_dat(str(path), cfg, psi)
     raw = np.fromfile(str(path), dtype=np.float64)
     assert raw.size == 5 * 6 * 4 * 12 * 3 * 2
     # block (i, j, k) = (2, 1, 0), unknown 7, component m = 0 sits where the reference puts it:
@@ -106,5 +107,8 @@ def test_reference_dat_roundtrip_and_layout(tmp_path):
     off = ((dst * 12 + l) * 3 + 1) * 2
     assert raw[off] == psi[1, k, j, i, l].real and raw[off + 1] == psi[1, k, j, i, l].imag
     assert np.all(raw.reshape(4, 6, 5, 12, 3, 2)[0] == 0)  # halo plane stays zero
-    back = states.read_reference_dat(str(path), dims, 3)
+    back = states.read_reference_dat(str(path), cfg)
     np.testing.assert_array_equal(back, psi)
+    # a file of another grid is refused, not silently misread
+    with pytest.raises(ValueError):
+        states.read_reference_dat(str(path), gpe.Config(lattice="bcc", ncomp=3, dims=(3, 4, 3)))

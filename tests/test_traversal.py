@@ -1,0 +1,163 @@
+"""Host-side guard of the sweep's index math: the wave decode of gpe_device.h (decode for one
+row per wave, decode_rows for x pairs over row groups) emulated with the constants of the
+library's own traversal plan (gpe_plan_traversal, computed by the same host code that sizes the
+launches).  For every wave of a sweep the unknown, plane, rows and x positions it touches lie
+inside the box or are masked, the fast divisions agree with exact integer division, the waves
+cover every (z, y, v, x) of the box exactly once, and both workgroup orders of consecutive
+sweeps (the XCD remap and its reversal) are permutations of the launch."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from spacetime_amd import gpe
+
+U32 = np.uint64(0xFFFFFFFF)
+
+
+class _Plan(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("nx", "ny", "nz", "V", "C", "x_per_lane", "rows", "pairs", "nseg",
+                                             "tile_rows", "row_groups", "ntile", "xcd_remap")] + [
+        ("lx", ctypes.c_longlong), ("waves", ctypes.c_longlong), ("blocks", ctypes.c_int)] + [
+        (n, ctypes.c_uint * 3) for n in ("div_nseg", "div_V", "div_rows", "div_nz", "div_pairs")]
+
+
+def _plan(cfg, ty, remap, xpl):
+    lib = gpe.load_library()
+    f = lib.gpe_plan_traversal
+    f.argtypes = [ctypes.POINTER(gpe._Config), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_Plan)]
+    p = _Plan()
+    rc = f(cfg.to_c(), ty, remap, xpl, ctypes.byref(p))
+    return rc, p
+
+
+def _fdiv(n, div):
+    """The device's n / d as multiply-high and shifts (gpe_device.h fast_div), on uint32 n."""
+    d, m, l = (int(x) for x in div)
+    n = n.astype(np.uint64)
+    if d == 1:
+        return n
+    t = (n * np.uint64(m)) >> np.uint64(32)
+    return (t + ((n - t) >> np.uint64(1))) >> np.uint64(l - 1)
+
+
+def _logical_block(b, nb):
+    q, r, k = nb >> 3, nb & 7, b & 7
+    return k * q + np.minimum(k, r) + (b >> 3)
+
+
+def _reverse_in_xcd(b, nb):
+    q, r, k = nb >> 3, nb & 7, b & 7
+    length = q + (k < r)
+    return k + 8 * (length - 1 - (b >> 3))
+
+
+def _decode_all(p):
+    """Every (workgroup, wave, lane) of a sweep: arrays of v, z, y, x, live flag."""
+    nb = p.blocks
+    b = np.arange(nb, dtype=np.int64)
+    lb = _logical_block(b, nb) if p.xcd_remap else b
+    w = (lb[:, None] * 4 + np.arange(4)[None, :]).reshape(-1).astype(np.uint64)
+    assert int(w.max()) < 2 ** 32
+    q0 = _fdiv(w, p.div_nseg)
+    xseg = (w - q0 * np.uint64(p.nseg)).astype(np.int64)
+    q1 = _fdiv(q0, p.div_V)
+    v = (q0 - q1 * np.uint64(p.V)).astype(np.int64)
+    q2 = _fdiv(q1, p.div_rows)
+    q3 = _fdiv(q2, p.div_nz)
+    z = (q2 - q3 * np.uint64(p.nz)).astype(np.int64)
+    # exact divisions for comparison
+    wi = w.astype(np.int64)
+    assert np.array_equal(q0.astype(np.int64), wi // p.nseg)
+    assert np.array_equal(q1.astype(np.int64), (wi // p.nseg) // p.V)
+    assert np.array_equal(q2.astype(np.int64), ((wi // p.nseg) // p.V) // p.row_groups)
+    lane = np.arange(64, dtype=np.int64)
+    if p.rows == 1:
+        yin = (q1 - q2 * np.uint64(p.tile_rows)).astype(np.int64)
+        y0 = q3.astype(np.int64) * p.tile_rows + yin
+        valid = (wi < p.waves) & (y0 < p.ny)
+        x = ((xseg[:, None] * 64 + lane[None, :]) * p.x_per_lane)
+        r = np.zeros_like(x)
+    else:
+        yg = (q1 - q2 * np.uint64(p.row_groups)).astype(np.int64)
+        y0 = q3.astype(np.int64) * p.tile_rows + yg * p.rows
+        valid = (wi < p.waves) & (y0 < p.ny)
+        slot = (xseg[:, None] * 64 + lane[None, :]).astype(np.uint64)
+        rr = _fdiv(slot.reshape(-1), p.div_pairs).reshape(slot.shape)
+        assert np.array_equal(rr.astype(np.int64), slot.astype(np.int64) // p.pairs)
+        r = rr.astype(np.int64)
+        x = 2 * (slot.astype(np.int64) - r * p.pairs)
+    return v, z, y0, x, r, valid
+
+
+CASES = [  # lattice, ncomp, precision, dims
+    ("bcc", 3, gpe.FP32, (64, 20, 7)), ("bcc", 3, gpe.FP32, (130, 9, 5)), ("bcc", 1, gpe.FP32, (192, 6, 3)),
+    ("bcc", 5, gpe.FP64, (70, 11, 4)), ("cube", 5, gpe.FP32, (33, 13, 6)), ("fcc", 3, gpe.FP32, (7, 17, 5)),
+    ("a15", 1, gpe.FP32, (57, 10, 3)), ("c15", 5, gpe.FP32, (29, 6, 3)), ("z", 3, gpe.FP32, (66, 9, 4)),
+    ("fcc", 1, gpe.FP32, (160, 5, 3)), ("cube", 3, gpe.FP32, (293, 4, 2)),
+]
+
+
+@pytest.mark.parametrize("lat,ncomp,precision,dims", CASES)
+def test_decode_stays_in_range_and_covers_the_box(lat, ncomp, precision, dims):
+    cfg = gpe.Config(lattice=lat, ncomp=ncomp, precision=precision, dims=dims)
+    nx, ny, nz = dims
+    xpls = (1, 2) if precision == gpe.FP32 else (1,)
+    checked = 0
+    for xpl in xpls:
+        for ty in (1, 2, 3, 4, 8, 16, 32, 64):
+            for remap in (0, 1):
+                rc, p = _plan(cfg, ty, remap, xpl)
+                assert rc == 0
+                assert p.blocks * 4 >= p.waves and p.tile_rows <= ny and p.tile_rows % p.rows == 0
+                assert p.lx >= nx and p.lx % 2 == 0 or p.lx == nx
+                v, z, y0, x, r, valid = _decode_all(p)
+                V = p.V
+                # wave-uniform coordinates of valid waves are inside the box
+                assert np.all((v[valid] >= 0) & (v[valid] < V))
+                assert np.all((z[valid] >= 0) & (z[valid] < nz))
+                assert np.all((y0[valid] >= 0) & (y0[valid] < ny))
+                count = np.zeros((nz, ny, V, nx), dtype=np.int32)
+                if p.rows == 1:
+                    live = valid[:, None] & (x < nx)
+                    zz, yy, vv = (np.broadcast_to(a[:, None], x.shape) for a in (z, y0, v))
+                    for e in range(xpl):
+                        ok = live & (x + e < nx)
+                        np.add.at(count, (zz[ok], yy[ok], vv[ok], (x + e)[ok]), 1)
+                else:
+                    nrow = np.minimum(ny - y0, p.rows)
+                    live = valid[:, None] & (r < nrow[:, None]) & (x < nx)
+                    # the lane's row (clamped into the box for masked lanes) is inside the box
+                    rr = np.where(r < nrow[:, None], r, nrow[:, None] - 1)
+                    assert np.all(y0[valid, None] + rr[valid] < ny)
+                    zz, vv = (np.broadcast_to(a[:, None], x.shape) for a in (z, v))
+                    yy = y0[:, None] + r
+                    for e in range(2):
+                        ok = live & (x + e < nx)
+                        np.add.at(count, (zz[ok], yy[ok], vv[ok], (x + e)[ok]), 1)
+                    # an x pair that starts inside the line ends inside its padding at most
+                    assert np.all(x[live] + 1 < p.lx)
+                assert count.min() == 1 and count.max() == 1, (ty, remap, xpl)
+                checked += 1
+    assert checked >= 16
+
+
+@pytest.mark.parametrize("nb", [1, 7, 8, 9, 63, 64, 1000, 4097])
+def test_workgroup_orders_are_permutations(nb):
+    b = np.arange(nb)
+    assert np.array_equal(np.sort(_logical_block(b, nb)), b)
+    rev = _reverse_in_xcd(b, nb)
+    assert np.array_equal(np.sort(rev), b)
+    assert np.all(rev % 8 == b % 8)  # a reversed workgroup stays on its XCD
+    # within each XCD's range the remapped order runs backwards
+    lb = _logical_block(rev, nb)
+    for k in range(min(8, nb)):
+        sel = b % 8 == k
+        assert np.all(np.diff(lb[sel]) < 0) or sel.sum() <= 1
+
+
+def test_plan_rejects_bad_arguments():
+    cfg = gpe.Config(lattice="bcc", ncomp=3, precision=gpe.FP64, dims=(8, 8, 8))
+    assert _plan(cfg, 8, 1, 2)[0] == -22  # FP64 has one x per lane
+    assert _plan(cfg, 0, 1, 1)[0] == -22
+    assert _plan(gpe.Config(lattice="bcc", dims=(0, 8, 8)), 8, 1, 1)[0] == -22
