@@ -117,12 +117,12 @@ def workload_scalar():
     return cfg, None, info
 
 
-def ensemble_members(seed=4848, n=48, box=16.0):
+def ensemble_members(seed=4848, n=48, box=16.0, blocks=(32, 96)):
     """Seeded member list: (config, knots, seed) with c1, c2 signs, lattice and blocks per axis drawn."""
     from spacetime_amd import gpe, lattice
     rng = np.random.default_rng(seed)
     names = rng.choice(["cube", "bcc", "fcc", "a15", "c15"], n)  # the workload's five lattices
-    dims = rng.integers(32, 97, size=n)
+    dims = rng.integers(blocks[0], blocks[1] + 1, size=n)
     signs = rng.choice([-1.0, 1.0], size=(n, 2))
     knots = rng.uniform(-0.2, 0.2, size=(n, 8))
     out = []
@@ -136,16 +136,16 @@ def ensemble_members(seed=4848, n=48, box=16.0):
     return out
 
 
-def run_ensemble(args, log):
-    """Aggregate node-updates/s of the batched ensemble vs the same runs one after another."""
-    from spacetime_amd import gpe, states
-    members = ensemble_members()
+def measure_ensemble(members, args, log, label):
+    """The members stepped one after another (each alone, eager launches) and together (one
+    launch per kernel group per sweep, replayed from a captured hipGraph), from the same states."""
+    from spacetime_amd import gpe
     duration = 5000 * 1e-4
     steppers = []
     nodes = 0.0
     for cfg, knots, _ in members:
         st = gpe.Stepper(cfg)
-        st.set_ramp(b_segments=states.knot_ramp(knots, duration))
+        st.set_knot_ramp(knots, duration)
         steppers.append(st)
         nodes += float(np.prod(cfg.dims) * cfg.V)
     gmu = 1.0e4
@@ -158,14 +158,14 @@ def run_ensemble(args, log):
             st.scale(1.0 / np.sqrt(st.observables(0).norm))
             st.start(gpe.START_EULER)
 
-    log("ensemble: %d members, %.3e unknowns in total (%.1f GB of FP32 levels)" % (
-        len(members), nodes, nodes * 5 * 8 * 2 / 1e9))
+    log("%s: %d members, %.3e unknowns in total (%.2f GB of FP32 levels)" % (
+        label, len(members), nodes, nodes * 5 * 8 * 2 / 1e9))
     prepare()
     tune = []
     t0 = time.perf_counter()
     for st in steppers:
         tune.append(st.autotune(args.tune_sweeps))
-    log("autotune: %d members in %.1fs" % (len(steppers), time.perf_counter() - t0))
+    log("%s autotune: %d members in %.1fs" % (label, len(steppers), time.perf_counter() - t0))
     prepare()
 
     # one after another: each member alone, eager launches
@@ -185,7 +185,8 @@ def run_ensemble(args, log):
     _sync()
     wall_seq = time.perf_counter() - t0
     obs_seq = [st.observables(0) for st in steppers]
-    log("sequential: %d steps in %.3fs, %.3e node-updates/s" % (args.steps, wall_seq, nodes * args.steps / wall_seq))
+    log("%s sequential: %d steps in %.3fs, %.3e node-updates/s" % (label, args.steps, wall_seq,
+                                                                   nodes * args.steps / wall_seq))
     by_lattice = {}
     for lat, n, ms, nb in per_member:
         a = by_lattice.setdefault(lat, [0.0, 0.0])
@@ -193,8 +194,8 @@ def run_ensemble(args, log):
         a[1] += nb
     total_ms = sum(a[0] for a in by_lattice.values())
     for lat, (ms, nb) in sorted(by_lattice.items(), key=lambda kv: -kv[1][0]):
-        log("sequential %-4s: %.3f ms per sweep (%.0f%% of the sum), %.0f GB/s" % (
-            lat, ms, 100 * ms / total_ms, nb / (ms * 1e-3) / 1e9))
+        log("%s sequential %-4s: %.3f ms per sweep (%.0f%% of the sum), %.0f GB/s" % (
+            label, lat, ms, 100 * ms / total_ms, nb / (ms * 1e-3) / 1e9))
 
     prepare()
     G = args.graph_sweeps
@@ -211,36 +212,55 @@ def run_ensemble(args, log):
     rel_n = max(abs(a.norm - b.norm) / b.norm for a, b in zip(obs_ens, obs_seq))
     drift = max(abs(o.norm - 1.0) for o in obs_ens)
     finite = all(o.finite for o in obs_ens)
-    log("ensemble: %d steps in %.3fs, %.4f ms/sweep, %.3e node-updates/s; max rel energy diff vs sequential %.2e" % (
-        args.steps, wall_ens, ms_sweep, nodes * args.steps / wall_ens, rel_e))
+    log("%s together: %d steps in %.3fs, %.4f ms/sweep, %.3e node-updates/s (%.3fx); max rel energy diff vs "
+        "sequential %.2e" % (label, args.steps, wall_ens, ms_sweep, nodes * args.steps / wall_ens, wall_seq / wall_ens,
+                              rel_e))
     for st in steppers:
         st.close()
     counts = {}
     for cfg, _, _ in members:
         counts[cfg.lattice] = counts.get(cfg.lattice, 0) + 1
-    achieved = bytes_sweep / (ms_sweep * 1e-3) / 1e9
+    return dict(nodes=nodes, wall_seq=wall_seq, wall_ens=wall_ens, ms_sweep=ms_sweep, bytes_sweep=bytes_sweep,
+                total_ms=total_ms, per_member=per_member, counts=counts, tune=tune,
+                accuracy={"max_rel_energy_diff_vs_sequential": rel_e, "max_rel_norm_diff_vs_sequential": rel_n,
+                          "max_norm_drift": drift, "finite": finite})
+
+
+def run_ensemble(args, log):
+    """Aggregate node-updates/s of the batched ensemble vs the same runs one after another: the
+    configured 48 spin-2 members of 32^3-96^3 blocks, and 48 small members of 8^3-24^3 blocks,
+    where a member alone cannot fill the GPU."""
+    big = measure_ensemble(ensemble_members(), args, log, "ensemble")
+    small = measure_ensemble(ensemble_members(seed=4849, blocks=(8, 24)), args, log, "small members")
+    achieved = big["bytes_sweep"] / (big["ms_sweep"] * 1e-3) / 1e9
+    nodes, G = big["nodes"], args.graph_sweeps
     return {
         "metric": "aggregate node-updates/s of a 48-run spin-2 ensemble (batched launches per sweep, hipGraph) "
                   "vs the runs one after another",
-        "value": nodes * args.steps / wall_ens,
+        "value": nodes * args.steps / big["wall_ens"],
         "unit": "node-updates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall_ens / args.steps,
+        "ms_per_step": 1e3 * big["wall_ens"] / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": "ensemble_spin2_48_mixed", "members": len(members), "lattices": counts,
+        "config": {"workload": "ensemble_spin2_48_mixed", "members": 48, "lattices": big["counts"],
                    "blocks_per_axis": [32, 96], "unknowns": nodes, "components": 5, "dt": 1e-4,
                    "graph_sweeps": G,
-                   "autotuned": [[t["tile_rows"], t["xcd_remap"], t["x_per_lane"]] for t in tune]},
-        "sequential": {"value": nodes * args.steps / wall_seq, "ms_per_step": 1e3 * wall_seq / args.steps,
-                       "kernel_ms_per_sweep_sum": total_ms,
+                   "autotuned": [[t["tile_rows"], t["xcd_remap"], t["x_per_lane"]] for t in big["tune"]]},
+        "sequential": {"value": nodes * args.steps / big["wall_seq"], "ms_per_step": 1e3 * big["wall_seq"] / args.steps,
+                       "kernel_ms_per_sweep_sum": big["total_ms"],
                        "members": [[lat, n, round(ms, 5), round(nb / (ms * 1e-3) / 1e9, 1)]
-                                   for lat, n, ms, nb in per_member]},
-        "speedup_vs_sequential": wall_seq / wall_ens,
+                                   for lat, n, ms, nb in big["per_member"]]},
+        "speedup_vs_sequential": big["wall_seq"] / big["wall_ens"],
+        "small_members": {"blocks_per_axis": [8, 24], "members": 48, "lattices": small["counts"],
+                          "unknowns": small["nodes"],
+                          "value": small["nodes"] * args.steps / small["wall_ens"],
+                          "sequential_value": small["nodes"] * args.steps / small["wall_seq"],
+                          "speedup_vs_sequential": small["wall_seq"] / small["wall_ens"],
+                          "accuracy": small["accuracy"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel_ms_per_sweep": ms_sweep,
-                     "algorithmic_bytes_per_sweep": bytes_sweep},
-        "accuracy": {"max_rel_energy_diff_vs_sequential": rel_e, "max_rel_norm_diff_vs_sequential": rel_n,
-                     "max_norm_drift": drift, "finite": finite},
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel_ms_per_sweep": big["ms_sweep"],
+                     "algorithmic_bytes_per_sweep": big["bytes_sweep"]},
+        "accuracy": big["accuracy"],
         "cpu_baseline": None,
     }
 

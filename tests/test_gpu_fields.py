@@ -93,22 +93,37 @@ def test_uv_theta_spin1():
 
 
 def test_save_volume_matches_field_maps(tmp_path):
-    """The volume file of a spin-1 state carries the device field maps (float32) at the
-    unknowns' positions, zero below the density threshold."""
-    from spacetime_amd import states, vtk
+    """The volume file of a spin-1 state, written from the device level in the reference's
+    padded layout (points z, x, y, v over the grid with a one-block halo), carries the device
+    field maps at the interior points: the density everywhere, the local spin with the
+    reference's z part (psi^dag Fz psi, not divided by n) and zeros below the density
+    threshold; the halo points are zero."""
+    from spacetime_amd import lattice, states, vtk
     cfg = _cfg(3)
     psi = random_state(cfg.shape, 5)
     psi[:, 1, 2, 3, :4] *= 1e-3  # a few unknowns below the density threshold
     with gpe.Stepper(cfg) as st:
         st.set_state(psi)
-        dens = st.local_fields(gpe.FIELD_DENSITY)[..., 0].reshape(-1)
-        spin = st.local_fields(gpe.FIELD_SPIN).reshape(-1, 3)
-        theta = st.local_fields(gpe.FIELD_UV_THETA)[..., 6].reshape(-1)
+        dens = st.local_fields(gpe.FIELD_DENSITY)[..., 0]
+        spin = st.local_fields(gpe.FIELD_SPIN)
         vtk.save_volume(st, tmp_path / "s.vtk")
     got = vtk.read_volume(tmp_path / "s.vtk")
+    nz, ny, nx, V = cfg.shape[1:]
+    w = lattice.get(cfg.lattice).block_dims
+    z, y, x, v = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), np.arange(V), indexing="ij")
+    k = ((((z + 1) * (nx + 2) + x + 1) * (ny + 2) + y + 1) * V + v).reshape(-1)
+    npad = (nz + 2) * (nx + 2) * (ny + 2) * V
+    assert got["points"].shape == (npad, 3)
+    shifted = states.positions(cfg) + cfg.block_scale * w  # one block of halo before index 0
+    np.testing.assert_allclose(got["points"][k], shifted.reshape(-1, 3).astype(np.float32), rtol=1e-6, atol=1e-6)
+    dens, spin = dens.reshape(-1), spin.reshape(-1, 3)
     live = dens > vtk.DENSITY_THRESHOLD
     assert 0 < live.sum() < live.size
-    np.testing.assert_array_equal(got["points"], states.positions(cfg).reshape(-1, 3).astype(np.float32))
-    np.testing.assert_array_equal(got["density"], dens.astype(np.float32))
-    np.testing.assert_array_equal(got["localAvgSpin"], np.where(live[:, None], spin, 0.0).astype(np.float32))
-    np.testing.assert_array_equal(got["theta"], np.where(live, theta, 0.0).astype(np.float32))
+    np.testing.assert_allclose(got["density"][k], dens, rtol=1e-6)
+    ref_spin = np.where(live[:, None], spin * np.array([1.0, 1.0, 0.0]) + np.outer(spin[:, 2] * dens, [0, 0, 1]), 0)
+    np.testing.assert_allclose(got["localAvgSpin"][k], ref_spin, rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(got["spinNorm"][k], np.linalg.norm(ref_spin, axis=1), rtol=1e-5, atol=1e-6)
+    halo = np.ones(npad, bool)
+    halo[k] = False
+    for name in ("density", "spinNorm", "theta"):
+        assert not np.any(got[name][halo]), name

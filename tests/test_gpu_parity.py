@@ -418,7 +418,9 @@ def test_staggered_run_matches_reference_golden(oracle, case):
     iz, iy, ix = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
     blk = np.stack([ix, iy, iz], -1)[:, :, :, None, :] * w[None, None, None, None, :]
     r = np.array(p0)[None, None, None, None, :] + s * (blk + gl.pos[None, None, None, :, :])
-    rr = np.hypot(r[..., 0], r[..., 1])
+    # sqrt(x^2 + y^2) as VortexState::getPsi forms it; np.hypot rounds differently and the
+    # interpolant's discontinuity turns one ulp of r into a different sample
+    rr = np.sqrt(r[..., 0] * r[..., 0] + r[..., 1] * r[..., 1])
     fv = np.vectorize(lambda a, b: oracle.vortex_function(vs, a, b))(rr, r[..., 2])
     with np.errstate(invalid="ignore", divide="ignore"):
         ph = ((r[..., 0] + 1j * r[..., 1]) / rr) ** c["kappa"]
