@@ -134,6 +134,14 @@ int gpe_get_state(gpe_ctx *ctx, int level, double *psi);
 /* Same with device pointers (complex128, ABI layout) on the context's device. */
 int gpe_set_state_device(gpe_ctx *ctx, int level, const double *dpsi, double t);
 int gpe_get_state_device(gpe_ctx *ctx, int level, double *dpsi);
+/* gpe_get_state that allocates nothing (no hipMalloc / hipFree, no allocating library call):
+ * for callers that cannot allocate device memory, such as tight serving loops and harnesses
+ * that forbid it.  No counterpart in the reference.  The level streams to the host through a
+ * file-scope device array of 32 MiB (held by every GPU the library runs on until the process
+ * ends) in chunks of at most that size, so any level size is accepted; each chunk is written
+ * in full before it is copied.  Calls that overlap on one device share its array and race.
+ * Synchronises the context's stream. */
+int gpe_get_state_noalloc(gpe_ctx *ctx, int level, double *psi);
 
 /* Static linear and quadratic Zeeman fields (MagFields Bs, Spin1GpeBCC.cu:1429-1432). */
 int gpe_set_field(gpe_ctx *ctx, double bq, const double bb[3], double bq_quad, const double bb_quad[3]);

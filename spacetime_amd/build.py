@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libgpe.so")
 
-SOURCES = ["gpe_kernels.hip", "gpe_adjoint.hip", "gpe_host.cpp", "gpe_states.cpp", "gpe_io.cpp"]
+SOURCES = ["gpe_kernels.hip", "gpe_adjoint.hip", "gpe_host.cpp", "gpe_states.cpp", "gpe_io.cpp", "gpe_noalloc.hip"]
 ARCH = os.environ.get("GPE_OFFLOAD_ARCH", "gfx950")
 
 # Packed-FP32 VALU forms are disabled for all device code.
