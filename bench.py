@@ -364,8 +364,12 @@ def run_lattices(args, log):
 
 def run_adjoint(args, log):
     """Spin-1 BCC 64^3 discrete adjoint: gradients of |<target|psi(T)>|^2 with respect to the 16
-    Bz knots and the 3 trap frequencies, FP32 sweeps with FP64 accumulation, checkpoints every
-    200 sweeps; checked against central finite differences of FP64 forward runs (h = 1e-4)."""
+    Bz knots and the 3 trap frequencies, checkpoints every 200 sweeps, in three forms: an FP32
+    context (FP32 sweeps and states, FP64 accumulation), the same context with fp64_trajectory
+    (trajectory and adjoint state in FP64), and an FP64 context; checked against central finite
+    differences of FP64 forward runs (h = 1e-4 and h / 2, Richardson-combined).  The start state
+    and the target are rounded to FP32 first, so every form and the finite differences solve the
+    problem the FP32 context holds."""
     from spacetime_amd import gpe, states
     from spacetime_amd.adjoint import RampTrapProblem
     import dataclasses
@@ -378,16 +382,18 @@ def run_adjoint(args, log):
     duration = nsweeps * cfg.dt
     knots = np.random.default_rng(2024).uniform(-0.1, 0.1, 16)
     ref_knots = np.random.default_rng(4000).uniform(-0.1, 0.1, 16)
-    psi0 = prepare_state(cfg, knots, args.itp_steps, args.itp_dt, log)
+    fp32_round = lambda a: a.astype(np.complex64).astype(np.complex128)  # noqa: E731
+    psi0 = fp32_round(prepare_state(cfg, knots, args.itp_steps, args.itp_dt, log))
     with gpe.Stepper(cfg64) as st:  # the target: the same start under the seeded reference ramp
         st.set_ramp(q_segments=[(4e-3, 1e30, 0)], b_segments=states.knot_ramp(ref_knots, duration))
         st.set_state(psi0)
         st.start(gpe.START_EULER)
         st.step(nsweeps - 1)
-        target = st.get_state(0)
+        target = fp32_round(st.get_state(0))
     out = {}
-    for name, c in (("fp32", cfg), ("fp64", cfg64)):
-        prob = RampTrapProblem(c, psi0, nsweeps, K, duration, objective=gpe.OBJ_OVERLAP, target=target, bq=4e-3)
+    for name, c, f64 in (("fp32", cfg, False), ("fp32_fp64_trajectory", cfg, True), ("fp64", cfg64, False)):
+        prob = RampTrapProblem(c, psi0, nsweeps, K, duration, objective=gpe.OBJ_OVERLAP, target=target, bq=4e-3,
+                               fp64_trajectory=f64)
         prob.value_and_grad(knots, c.trap)  # warm-up (allocations, autotune-free)
         t0 = time.perf_counter()
         J, gk, gt = prob.value_and_grad(knots, c.trap)
@@ -422,9 +428,10 @@ def run_adjoint(args, log):
         "forward_ms": r32["forward_ms"], "backward_ms": r32["backward_ms"],
         "fp64_form": {"value": out["fp64"]["wall_s"], "forward_ms": out["fp64"]["forward_ms"],
                       "backward_ms": out["fp64"]["backward_ms"]},
-        "accuracy": {"J_fp32": r32["J"], "J_fp64": out["fp64"]["J"],
-                     "grad_fp32": np.concatenate([r32["grad_knots"], r32["grad_trap"]]).tolist(),
-                     "grad_fp64": np.concatenate([out["fp64"]["grad_knots"], out["fp64"]["grad_trap"]]).tolist()},
+        "fp32_fp64_trajectory_form": {"value": out["fp32_fp64_trajectory"]["wall_s"],
+                                      "forward_ms": out["fp32_fp64_trajectory"]["forward_ms"],
+                                      "backward_ms": out["fp32_fp64_trajectory"]["backward_ms"]},
+        "accuracy": {"J_" + k: r["J"] for k, r in out.items()},
         "roofline": None, "cpu_baseline": None,
     }
     if args.no_fd:  # timing only (profiling runs): no finite-difference check
@@ -451,17 +458,23 @@ def run_adjoint(args, log):
     fd.close()
     g_fd = (4 * g_h2 - g_h) / 3
     log("finite differences: 76 FP64 forward runs in %.1f s" % (time.perf_counter() - t0))
-    scale = float(np.abs(g_fd).max())
+    # errors relative to the largest finite difference of each group: the knot gradients
+    # (~1e-8) are two orders below the trap gradients (~1e-6)
+    groups = {"knots": slice(0, 16), "trap": slice(16, 19)}
     err, err_raw = {}, {}
     for name, r in out.items():
         g = np.concatenate([r["grad_knots"], r["grad_trap"]])
-        err[name] = float(np.abs(g - g_fd).max() / scale)
-        err_raw[name] = float(np.abs(g - g_h).max() / scale)
-    log("max |adjoint - FD| / max |FD| (Richardson FD): fp32 %.2e, fp64 %.2e; against plain h = 1e-4: fp32 %.2e, "
-        "fp64 %.2e" % (err["fp32"], err["fp64"], err_raw["fp32"], err_raw["fp64"]))
-    line["accuracy"].update({"max_rel_grad_err_fp32_vs_fd": err["fp32"], "max_rel_grad_err_fp64_vs_fd": err["fp64"],
-                             "max_rel_grad_err_vs_plain_fd_h1e-4": err_raw, "grad_fd_richardson": g_fd.tolist(),
-                             "grad_fd_h1e-4": g_h.tolist()})
+        err[name] = {k: float(np.abs(g[sl] - g_fd[sl]).max() / np.abs(g_fd[sl]).max()) for k, sl in groups.items()}
+        err_raw[name] = {k: float(np.abs(g[sl] - g_h[sl]).max() / np.abs(g_h[sl]).max()) for k, sl in groups.items()}
+        log("%s: max |adjoint - FD| / max |FD|: knots %.2e, trap %.2e (plain h = 1e-4: %.2e, %.2e)" % (
+            name, err[name]["knots"], err[name]["trap"], err_raw[name]["knots"], err_raw[name]["trap"]))
+    line["accuracy"].update({
+        "max_rel_grad_err_fp32_vs_fd": max(err["fp32"].values()),
+        "max_rel_grad_err_fp32_fp64_trajectory_vs_fd": max(err["fp32_fp64_trajectory"].values()),
+        "max_rel_grad_err_fp64_vs_fd": max(err["fp64"].values()),
+        "per_group": err, "per_group_vs_plain_fd_h1e-4": err_raw,
+        "grad": {k: np.concatenate([r["grad_knots"], r["grad_trap"]]).tolist() for k, r in out.items()},
+        "grad_fd_richardson": g_fd.tolist(), "grad_fd_h1e-4": g_h.tolist()})
     return line
 
 

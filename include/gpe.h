@@ -165,6 +165,20 @@ int gpe_itp(gpe_ctx *ctx, int nsteps, double dt_itp);
  * the integrate trees (Spin1GpeBCC.cu:191-271,328-392,955-1020). */
 int gpe_observables(gpe_ctx *ctx, int level, gpe_observables_t *out);
 
+/* Maximum over the unknowns of a level of the local Hamiltonian bound: with p = V(r) + c0 n
+ * and B = B(r, t) + c1 psi^dag F psi, max(p + |Bx|/sqrt2, p + |By|/sqrt2, p + |Bz|) for spin-1,
+ * p for a scalar state (the GPBlocks dt rule's max potential).  Replaces maxHamilton +
+ * reduceMax (Spin1GpeBCC.cu:144-189,1022-1037; GPBlocks.cpp:488-495).  GPE_EINVAL for spin-2. */
+int gpe_max_hamiltonian(gpe_ctx *ctx, int level, double *out);
+
+/* sum over unknowns of dv |a|^2 |psi - a|^2 with a = (phase_re + i phase_im) ref: the
+ * reference's error against an analytic state, weightedDiff + integrate
+ * (Spin1GpeBCC.cu:436-461,1620-1633; RelativisticSpin2Gpe.cu:227).  ref: complex128, ABI
+ * layout, a host array (copied to the device per call) or, for _device, device memory. */
+int gpe_weighted_diff(gpe_ctx *ctx, int level, const double *ref, double phase_re, double phase_im, double *out);
+int gpe_weighted_diff_device(gpe_ctx *ctx, int level, const double *dref, double phase_re, double phase_im,
+                             double *out);
+
 /* Per-unknown field maps of a level, copied to `out` in ABI order (z, y, x, v) with
  * gpe_field_width(kind) doubles per unknown.  Replaces localAvgSpinAndDensity and uvTheta
  * (Spin1GpeBCC.cu:235-326), which feed the reference's VTK/state writers.

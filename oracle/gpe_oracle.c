@@ -500,6 +500,73 @@ void orc_magnetization(const orc_lattice *L, const orc_params *p, const double *
     }
 }
 
+double orc_max_hamiltonian(const orc_lattice *L, const orc_params *p, double t, const double *psi_)
+{
+    /* maxHamilton + reduceMax (Spin1GpeBCC.cu:144-189,1022-1037): per unknown p = V + c0 n,
+     * B = B(r) + c1 psi^dag F psi (not divided by n), max(p + |Bx|/sqrt2, p + |By|/sqrt2,
+     * p + |Bz|); the scalar form is the max of V + g n of GPBlocks.cpp:488-495.  Spin-1 or
+     * scalar only (returns NaN for spin-2, which has no such bound in the reference). */
+    const cplx *psi = (const cplx *)psi_;
+    const int V = L->V, C = p->ncomp;
+    const size_t nc = ncells(p) * V;
+    double best = -INFINITY;
+    int x, y, z, v, c;
+    if (C == 5) return NAN;
+    for (z = 0; z < p->nz; ++z)
+        for (y = 0; y < p->ny; ++y)
+            for (x = 0; x < p->nx; ++x)
+                for (v = 0; v < V; ++v) {
+                    const size_t i = ((size_t)(z * p->ny + y) * p->nx + x) * V + v;
+                    cplx own[3];
+                    double r[3], n = 0, h;
+                    orc_position(L, p, x, y, z, v, r);
+                    for (c = 0; c < C; ++c) {
+                        own[c] = psi[c * nc + i];
+                        n += cnorm2(own[c]);
+                    }
+                    const double pot = trap_potential(p, r, t) + p->c0 * n;
+                    h = pot;
+                    if (C == 3) {
+                        double Fv[3], B[3];
+                        spin_density(1, own, Fv);
+                        field_at(p->bq, p->bb, p->center, r, B);
+                        for (c = 0; c < 3; ++c) B[c] += p->c1 * Fv[c];
+                        h = fmax(pot + fabs(B[0]) / sqrt(2.0), fmax(pot + fabs(B[1]) / sqrt(2.0), pot + fabs(B[2])));
+                    }
+                    if (h > best) best = h;
+                }
+    return best;
+}
+
+double orc_weighted_diff(const orc_lattice *L, const orc_params *p, const double *psi_, const double *ref_,
+                         double phase_re, double phase_im)
+{
+    /* weightedDiff + integrate (Spin1GpeBCC.cu:436-461,1620-1633): sum dv |a|^2 |psi - a|^2
+     * over unknowns, a = phase * ref. */
+    const cplx *psi = (const cplx *)psi_, *ref = (const cplx *)ref_;
+    const int V = L->V, C = p->ncomp;
+    const size_t nc = ncells(p) * V, ncell = ncells(p);
+    const double s3 = p->block_scale * p->block_scale * p->block_scale;
+    const cplx ph = cx(phase_re, phase_im);
+    double total = 0;
+    size_t i;
+    int v, c;
+    for (v = 0; v < V; ++v) {
+        double acc = 0;
+        for (i = 0; i < ncell; ++i) {
+            double an = 0, dn = 0;
+            for (c = 0; c < C; ++c) {
+                const cplx a = cmul(ph, ref[c * nc + i * V + v]);
+                an += cnorm2(a);
+                dn += cnorm2(csub(psi[c * nc + i * V + v], a));
+            }
+            acc += an * dn;
+        }
+        total += acc * L->vol[v] * s3;
+    }
+    return total;
+}
+
 /* --------------------------------------------------------- vortex state ---- */
 
 void orc_vortex_init(int rsize, double rstep, double rplug, int zsize, double zstep, double *f, double *mu)

@@ -113,6 +113,11 @@ double orc_energy(const orc_lattice *L, const orc_params *p, double t, const dou
 double orc_expect_h(const orc_lattice *L, const orc_params *p, double t, const double *psi,
                     double *scratch);
 void orc_magnetization(const orc_lattice *L, const orc_params *p, const double *psi, double out[3]);
+/* max over unknowns of the local Hamiltonian bound (maxHamilton); NaN for spin-2. */
+double orc_max_hamiltonian(const orc_lattice *L, const orc_params *p, double t, const double *psi);
+/* sum dv |a|^2 |psi - a|^2, a = (phase_re + i phase_im) ref (weightedDiff + integrate). */
+double orc_weighted_diff(const orc_lattice *L, const orc_params *p, const double *psi, const double *ref,
+                         double phase_re, double phase_im);
 
 /* Position of unknown v of block (x, y, z). */
 void orc_position(const orc_lattice *L, const orc_params *p, int x, int y, int z, int v, double r[3]);

@@ -260,22 +260,24 @@ bool finish_geometry(Geometry &g)
     // x pairs: one row per wave leaves lanes idle unless a line holds a multiple of 128 values.
     // Lines of <= 64 (<= 32) values: the wave's pair slots spread over a group of 2 (4) rows of
     // the tile (a group size that divides the tile's rows).  Unit cells of <= 12 unknowns: the
-    // group of 1-8 rows with the best lane use when it fills at least 5% more lanes (BCC 70^3
-    // +9%, FCC 45^3 +7%, scalar FCC 160^3 +17%, spin-2 Cube 293^3 +8%, spin-1 BCC 192^3 +3%;
-    // scalar Cube 293^3 -2%).  Larger cells (A15, C15) lose 3-5% with such groups at 70-90
-    // blocks: their v-lines' shared neighbour lines are then used further apart in the traversal.
+    // group of 1-8 rows (dividing the tile's rows) with the best lane use when it fills at least
+    // 5% more lanes (BCC 70^3 +9%, FCC 45^3 +7%, scalar FCC 160^3 +17%, spin-2 Cube 293^3 +8%,
+    // spin-1 BCC 192^3 +3%; scalar Cube 293^3 -2%).  Larger cells (A15, C15, Z) take a group
+    // only where it packs >= 90% of the lanes: groups of 4 rows (85%) cost A15 82^3 and Z 68^3
+    // spin-2 5-7%, while 3 rows of A15's 41 pairs (96%, tiles of 6 or 12 rows) gain 6-8% and 7
+    // rows of Z's 34 pairs (93%, tiles of 14) 5%; the tile heights with such divisors are
+    // autotuner candidates.
     int rows = 1;
     if (g.xpl == 2) {
         const int want = g.nx <= 32 ? 4 : (g.nx <= 64 ? 2 : 1);
         while (rows < want && g.ty % (2 * rows) == 0) rows *= 2;
-        if (g.V <= 12) {
-            double best = lane_use(rows, g.pairs) + 0.05;
-            for (int r = 2; r <= 8; r *= 2)
-                if (g.ty % r == 0 && lane_use(r, g.pairs) > best + 1e-9) {
-                    best = lane_use(r, g.pairs);
-                    rows = r;
-                }
-        }
+        const double floor_use = g.V <= 12 ? 0.0 : 0.9;
+        double best = lane_use(rows, g.pairs) + 0.05;
+        for (int r = 2; r <= 8; ++r)
+            if (g.ty % r == 0 && lane_use(r, g.pairs) >= floor_use && lane_use(r, g.pairs) > best + 1e-9) {
+                best = lane_use(r, g.pairs);
+                rows = r;
+            }
     }
     g.rows = rows;
     g.tyr = g.ty / rows;
@@ -773,6 +775,70 @@ int gpe_observables(gpe_ctx *c, int level, gpe_observables_t *out)
     out->moment[1] = h[OBS_RY];
     out->moment[2] = h[OBS_RZ];
     return out->finite ? GPE_OK : fail(c, GPE_ERANGE, "non-finite value in the state");
+}
+
+// k_aux over a level with an optional device reference state; h[0] = maximum, h[1] = sum.
+static int aux_reduce(gpe_ctx *c, int level, const double *dref, double pre, double pim, double h[AUX_COUNT])
+{
+    const double t = c->t0 + (double)(c->sweeps - level) * sweep_dt(c);
+    AuxArgs aa;
+    aa.g = c->g;
+    aa.tb = tables(c);
+    aa.ph = c->ph;
+    aa.psi = c->level[level];
+    aa.ref = (const double2 *)dref;
+    aa.phase_re = pre;
+    aa.phase_im = pim;
+    aa.field = field_at(c, t);
+    aa.partials = c->d_partials;
+    hipError_t e = launch_aux(c->fp64, c->cfg.ncomp, aa, c->stream);
+    if (e == hipSuccess) e = launch_finalize(c->d_partials, c->g.blocks, AUX_COUNT, AUX_MAXHAM, c->d_scalars + 8, c->d_red, c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "local reduction");
+    HIPCALL(c, hipMemcpyAsync(h, c->d_scalars + 8, sizeof(double) * AUX_COUNT, hipMemcpyDeviceToHost, c->stream));
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+    return GPE_OK;
+}
+
+int gpe_max_hamiltonian(gpe_ctx *c, int level, double *out)
+{
+    if (!c) return GPE_EINVAL;
+    if (!out) return fail(c, GPE_EINVAL, "null output");
+    if (c->cfg.ncomp == 5) return fail(c, GPE_EINVAL, "the Hamiltonian bound is defined for scalar and spin-1 states");
+    int rc = check_level(c, level);
+    if (rc) return rc;
+    double h[AUX_COUNT];
+    if ((rc = aux_reduce(c, level, nullptr, 1.0, 0.0, h))) return rc;
+    *out = h[AUX_MAXHAM];
+    return std::isfinite(*out) ? GPE_OK : fail(c, GPE_ERANGE, "non-finite value in the state");
+}
+
+int gpe_weighted_diff_device(gpe_ctx *c, int level, const double *dref, double phase_re, double phase_im, double *out)
+{
+    if (!c) return GPE_EINVAL;
+    if (!dref || !out) return fail(c, GPE_EINVAL, "null reference state or output");
+    int rc = check_level(c, level);
+    if (rc) return rc;
+    double h[AUX_COUNT];
+    if ((rc = aux_reduce(c, level, dref, phase_re, phase_im, h))) return rc;
+    *out = h[AUX_WDIFF];
+    return GPE_OK;
+}
+
+int gpe_weighted_diff(gpe_ctx *c, int level, const double *ref, double phase_re, double phase_im, double *out)
+{
+    if (!c) return GPE_EINVAL;
+    if (!ref || !out) return fail(c, GPE_EINVAL, "null reference state or output");
+    int rc = check_level(c, level);
+    if (rc) return rc;
+    const size_t bytes = 2 * sizeof(double) * (size_t)c->cfg.ncomp * c->cfg.nx * c->cfg.ny * c->cfg.nz * c->g.V;
+    double *d = nullptr;
+    if (hipMalloc((void **)&d, bytes) != hipSuccess) return fail(c, GPE_ENOMEM, "reference state does not fit");
+    hipError_t e = hipMemcpyAsync(d, ref, bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) rc = gpe_weighted_diff_device(c, level, d, phase_re, phase_im, out);
+    else rc = hip_fail(c, e, "reference upload");
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    return rc;
 }
 
 int gpe_field_width(int kind)
@@ -1620,13 +1686,17 @@ int gpe_autotune(gpe_ctx *c, int trial_sweeps, int *tile_rows, int *xcd_remap, i
         HIPCALL(c, hipMemcpyAsync(trial.s1, c->level[1], lbytes, hipMemcpyDeviceToDevice, c->stream));
         int rc = upload_fields(c, c->sweeps, 1);
         if (rc) return rc;
-        const int tys[] = {1, 2, 4, 8, 16, 32};
+        // tile heights: powers of two, and multiples of 3, 5, 6 and 7 rows, which are
+        // candidates only where their row groups are not powers of two
+        const int tys[] = {1, 2, 4, 8, 16, 32, 3, 6, 12, 24, 5, 10, 20, 7, 14, 28};
         const int nxpl = pair_lanes_ok(c->fp64, c->g.deg, c->cfg.ncomp) ? 2 : 1;
         for (int xpl = 1; xpl <= nxpl; ++xpl)
         for (int ty : tys) {
             if (ty > c->g.ny && ty != 1) continue;
+            const bool pow2 = (ty & (ty - 1)) == 0;
             for (int remap = 0; remap < 2; ++remap) {
                 if (!set_traversal(c, ty, remap, xpl)) continue;  // too many workgroups: not a candidate
+                if (!pow2 && (c->gs.rows & (c->gs.rows - 1)) == 0) continue;
                 void *cur = trial.s0, *nxt = trial.s1;
                 for (int k = 0; k < trial_sweeps + 2; ++k) {
                     if (k == 2) HIPCALL(c, hipEventRecord(c->ev0, c->stream));

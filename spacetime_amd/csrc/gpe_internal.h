@@ -35,7 +35,7 @@ struct Geometry {
     FastDiv f_nseg, f_V, f_ty, f_nz;  // the decode's divisors (f_ty divides by tyr)
     FastDiv f_pairs;   // divides a lane's pair slot by `pairs` (rows > 1)
     int xpl;           // consecutive x values per lane (1, or 2 for the FP32 sweep)
-    int rows;          // y rows of a row group (1; 2, 4 or 8: x pairs spread over the group's rows)
+    int rows;          // y rows of a row group (1; 2-8: x pairs spread over the group's rows)
     int pairs;         // x pairs per row: ceil(nx / 2)
     int nseg;          // waves per x-line (rows == 1: ceil(nx / (64 xpl))) or per row group
                        // (rows > 1: ceil(rows * pairs / 64))
@@ -147,6 +147,21 @@ struct ObsArgs {
     double *partials;         // OBS_COUNT per workgroup
 };
 
+// Per-unknown bound of the local Hamiltonian (maxHamilton) and the weighted distance to a
+// reference state (weightedDiff), reduced per workgroup: partials[2 b] = max, [2 b + 1] = sum.
+enum { AUX_MAXHAM = 0, AUX_WDIFF, AUX_COUNT };
+
+struct AuxArgs {
+    Geometry g;
+    Tables tb;
+    Physics ph;
+    const void *psi;
+    const double2 *ref;       // reference state, complex128 ABI layout (null: no distance)
+    double phase_re, phase_im; // the reference is multiplied by this phase
+    FieldRec field;
+    double *partials;         // AUX_COUNT per workgroup
+};
+
 // Adjoint sweep: mu2 <- mu2 + coef G'(psi)^T mu1, plus per-workgroup parameter sensitivities.
 enum { ADJ_BX = 0, ADJ_BY, ADJ_BZ, ADJ_BQ, ADJ_WX, ADJ_WY, ADJ_WZ, ADJ_COUNT };
 
@@ -228,6 +243,7 @@ hipError_t launch_fill_tf(int fp64, const Geometry &g, const Tables &tb, const P
 
 // launchers (gpe_kernels.hip)
 hipError_t launch_sweep(int fp64, int ncomp, int mode, const SweepArgs &a, hipStream_t s);
+hipError_t launch_aux(int fp64, int ncomp, const AuxArgs &a, hipStream_t s);
 hipError_t launch_observables(int fp64, int ncomp, const ObsArgs &a, hipStream_t s);
 // sum `n` partial records of `width` doubles into out[width] (max for column maxcol, if >= 0)
 // scratch: FIN_SCRATCH doubles of device memory owned by the caller's context.

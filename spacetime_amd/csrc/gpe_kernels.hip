@@ -528,7 +528,7 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
     }
 }
 
-// X: the lane form (lane_form): 1 one x per lane, 2 x pairs, 3 x pairs over 2 or 4 y rows.
+// X: the lane form (lane_form): 1 one x per lane, 2 x pairs, 3 x pairs over a group of 2-8 y rows.
 template <typename R, int C, int DEG, int MODE, int X>
 __device__ __forceinline__ void sweep_body(const SweepArgs &a, int sweep, int b, int nb)
 {
@@ -724,6 +724,85 @@ __global__ __launch_bounds__(256) void k_observables(ObsArgs a)
         else
             r = (red[0][i] + red[1][i]) + (red[2][i] + red[3][i]);
         a.partials[(long long)blockIdx.x * OBS_COUNT + i] = r;
+    }
+}
+
+// Two local reductions over a level, one x per lane:
+//   max over unknowns of the local Hamiltonian bound (Spin1GpeBCC.cu:144-189): with
+//   p = V(r) + c0 n and B = B(r, t) + c1 <F> (<F> = psi^dag f psi, not divided by n),
+//   max(p + |Bx| / sqrt2, p + |By| / sqrt2, p + |Bz|); a scalar state gives V + c0 n;
+//   sum over unknowns of dv |a|^2 |psi - a|^2 with a = phase * ref (Spin1GpeBCC.cu:436-461
+//   with the volume factor of its integrate reduction, :1622-1631; spin-2: the same sum,
+//   RelativisticSpin2Gpe.cu:227).  Spin-2 has no Hamiltonian bound: its column holds V + c0 n.
+template <typename R, int C>
+__global__ __launch_bounds__(256) void k_aux(AuxArgs a)
+{
+    using V2 = vec2_t<R>;
+    const Geometry &g = a.g;
+    const WaveCoord wc = decode(g);
+    double hmax = -INFINITY, wsum = 0.0;
+    if (wc.valid && wc.x < g.nx) {
+        const int x = wc.x, v = wc.v, y = wc.y, z = wc.z;
+        const V2 *__restrict__ psi = (const V2 *)a.psi;
+        double pr[C], pi[C];
+        double n = 0;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const V2 o = psi[line_index(g, c, z, y, v) + x];
+            pr[c] = o.x;
+            pi[c] = o.y;
+            n += pr[c] * pr[c] + pi[c] * pi[c];
+        }
+        const Physics &ph = a.ph;
+        const double *pos = a.tb.pos64 + 3 * v;
+        const double sig = a.field.sigma;
+        const double rx = ph.p0[0] + sig * (ph.cell[0] * x + pos[0]);
+        const double ry = ph.p0[1] + sig * (ph.cell[1] * y + pos[1]);
+        const double rz = ph.p0[2] + sig * (ph.cell[2] * z + pos[2]);
+        double vtrap = 0;
+        if (a.field.t < ph.trap_off_time)
+            vtrap = 0.5 * (ph.trap2[0] * rx * rx + ph.trap2[1] * ry * ry + ph.trap2[2] * rz * rz) + ph.trap_offset;
+        const double p = vtrap + ph.c0 * n;
+        hmax = p;
+        if constexpr (C == 3) {
+            const double tr = (pr[0] * pr[1] + pi[0] * pi[1]) + (pr[1] * pr[2] + pi[1] * pi[2]);
+            const double ti = (pr[0] * pi[1] - pi[0] * pr[1]) + (pr[1] * pi[2] - pi[1] * pr[2]);
+            const double s2 = 1.4142135623730951;
+            const double Fx = s2 * tr, Fy = s2 * ti;
+            const double Fz = (pr[0] * pr[0] + pi[0] * pi[0]) - (pr[2] * pr[2] + pi[2] * pi[2]);
+            const double Bx = a.field.bq * (rx - ph.center[0]) + a.field.bb[0] + ph.c1 * Fx;
+            const double By = a.field.bq * (ry - ph.center[1]) + a.field.bb[1] + ph.c1 * Fy;
+            const double Bz = -2 * a.field.bq * (rz - ph.center[2]) + a.field.bb[2] + ph.c1 * Fz;
+            hmax = fmax(p + fabs(Bx) / s2, fmax(p + fabs(By) / s2, p + fabs(Bz)));
+        }
+        if (a.ref) {
+            const long long line = (long long)g.nx * g.V;
+            const long long plane = line * g.ny, comp = plane * g.nz;
+            const long long at = (long long)z * plane + (long long)y * line + (long long)x * g.V + v;
+            double an = 0, dn = 0;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const double2 r = a.ref[(long long)c * comp + at];
+                const double ar = a.phase_re * r.x - a.phase_im * r.y;
+                const double ai = a.phase_re * r.y + a.phase_im * r.x;
+                an += ar * ar + ai * ai;
+                dn += (pr[c] - ar) * (pr[c] - ar) + (pi[c] - ai) * (pi[c] - ai);
+            }
+            wsum = an * dn * a.tb.dv[v];
+        }
+    }
+    __shared__ double red[4][AUX_COUNT];
+    const double m = wave_max(hmax), s = wave_sum(wsum);
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6][AUX_MAXHAM] = m;
+        red[threadIdx.x >> 6][AUX_WDIFF] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a.partials[(long long)blockIdx.x * AUX_COUNT + AUX_MAXHAM] =
+            fmax(fmax(red[0][AUX_MAXHAM], red[1][AUX_MAXHAM]), fmax(red[2][AUX_MAXHAM], red[3][AUX_MAXHAM]));
+        a.partials[(long long)blockIdx.x * AUX_COUNT + AUX_WDIFF] =
+            (red[0][AUX_WDIFF] + red[1][AUX_WDIFF]) + (red[2][AUX_WDIFF] + red[3][AUX_WDIFF]);
     }
 }
 
@@ -1239,6 +1318,24 @@ static hipError_t obs_comp(int ncomp, const ObsArgs &a, hipStream_t s)
 hipError_t launch_observables(int fp64, int ncomp, const ObsArgs &a, hipStream_t s)
 {
     return fp64 ? obs_comp<double>(ncomp, a, s) : obs_comp<float>(ncomp, a, s);
+}
+
+template <typename R>
+static hipError_t aux_comp(int ncomp, const AuxArgs &a, hipStream_t s)
+{
+    dim3 grid(a.g.blocks), block(256);
+    switch (ncomp) {
+    case 1: hipLaunchKernelGGL((k_aux<R, 1>), grid, block, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_aux<R, 3>), grid, block, 0, s, a); break;
+    case 5: hipLaunchKernelGGL((k_aux<R, 5>), grid, block, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_aux(int fp64, int ncomp, const AuxArgs &a, hipStream_t s)
+{
+    return fp64 ? aux_comp<double>(ncomp, a, s) : aux_comp<float>(ncomp, a, s);
 }
 
 template <typename R, int C, int DEG>

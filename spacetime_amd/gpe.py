@@ -138,6 +138,9 @@ def load_library(path=LIB_PATH):
     lib.gpe_set_knot_ramp.argtypes = [vp, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double]
     lib.gpe_write_dat.argtypes = [vp, ctypes.c_int, ctypes.c_char_p]
     lib.gpe_read_dat.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_double]
+    lib.gpe_max_hamiltonian.argtypes = [vp, ctypes.c_int, _dp]
+    for name in ("gpe_weighted_diff", "gpe_weighted_diff_device"):
+        getattr(lib, name).argtypes = [vp, ctypes.c_int, vp, ctypes.c_double, ctypes.c_double, _dp]
     lib.gpe_save_checkpoint.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
     lib.gpe_load_checkpoint.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
                                         ctypes.POINTER(ctypes.c_int)]
@@ -428,6 +431,29 @@ class Stepper:
             self._check(rc, "gpe_observables")
         return Observables(o.norm, o.energy, o.expect_h, np.array(o.magnetization[:]), o.max_density, o.time,
                            bool(o.finite), o.spin, np.array(o.moment[:]))
+
+    def max_hamiltonian(self, level=0):
+        """Maximum local Hamiltonian bound of a level (gpe_max_hamiltonian; scalar and spin-1)."""
+        out = ctypes.c_double()
+        self._check(self.lib.gpe_max_hamiltonian(self._h, level, ctypes.byref(out)), "gpe_max_hamiltonian")
+        return out.value
+
+    def weighted_diff(self, ref, phase=1.0, level=0):
+        """sum dv |a|^2 |psi - a|^2, a = phase * ref (gpe_weighted_diff).  ref: a host array in
+        the ABI layout, or a device pointer (int) to one in complex128."""
+        out = ctypes.c_double()
+        phase = complex(phase)
+        if isinstance(ref, int):
+            rc = self.lib.gpe_weighted_diff_device(self._h, level, ctypes.c_void_p(ref), phase.real, phase.imag,
+                                                   ctypes.byref(out))
+            self._check(rc, "gpe_weighted_diff_device")
+        else:
+            a = np.ascontiguousarray(ref, dtype=np.complex128)
+            if a.shape != self.config.shape:
+                raise ValueError("reference shape %s, expected %s" % (a.shape, self.config.shape))
+            rc = self.lib.gpe_weighted_diff(self._h, level, a.ctypes.data, phase.real, phase.imag, ctypes.byref(out))
+            self._check(rc, "gpe_weighted_diff")
+        return out.value
 
     def local_fields(self, kind, level=0):
         """Per-unknown field map (gpe_local_fields), shaped (nz, ny, nx, V, width)."""

@@ -72,6 +72,10 @@ class Oracle:
         L.orc_expect_h.argtypes = [LP, P, ctypes.c_double, _dp, _dp]
         L.orc_expect_h.restype = ctypes.c_double
         L.orc_magnetization.argtypes = [LP, P, _dp, _dp]
+        L.orc_max_hamiltonian.argtypes = [LP, P, ctypes.c_double, _dp]
+        L.orc_max_hamiltonian.restype = ctypes.c_double
+        L.orc_weighted_diff.argtypes = [LP, P, _dp, _dp, ctypes.c_double, ctypes.c_double]
+        L.orc_weighted_diff.restype = ctypes.c_double
         L.orc_local_h.argtypes = [P, _dp, ctypes.c_double, _dp, _dp]
         L.orc_position.argtypes = [LP, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
         L.orc_run_leapfrog3.argtypes = [LP, P, RP, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp]
@@ -208,6 +212,14 @@ class Oracle:
         out = np.zeros(3)
         self.lib.orc_magnetization(ctypes.byref(L), ctypes.byref(p), self._c(psi), out.ctypes.data_as(_dp))
         return out
+
+    def max_hamiltonian(self, L, p, psi, t=0.0):
+        return self.lib.orc_max_hamiltonian(ctypes.byref(L), ctypes.byref(p), t, self._c(psi))
+
+    def weighted_diff(self, L, p, psi, ref, phase=1.0):
+        phase = complex(phase)
+        return self.lib.orc_weighted_diff(ctypes.byref(L), ctypes.byref(p), self._c(psi), self._c(ref), phase.real,
+                                          phase.imag)
 
     def local_h(self, p, r, psi_node, t=0.0):
         psi_node = np.ascontiguousarray(psi_node, dtype=np.complex128)

@@ -96,6 +96,10 @@ def test_wide_rows_span_several_waves(oracle, precision, tol, xpl, nx):
     ("bcc", 1, (128, 5, 3), (4, 1, 2)),  # one component, one row per wave; tile 2 holds only row 4
     ("a15", 1, (70, 6, 3), (2, 0, 2)),   # one component, 2-row tiles, no XCD remap
     ("z", 1, (66, 9, 3), (8, 1, 2)),     # one component over 33 pairs; tile 2 holds only row 8
+    ("a15", 5, (82, 7, 2), (6, 1, 2)),   # 3-row groups of 41 pairs; tile 2 holds only row 6
+    ("z", 3, (68, 16, 2), (14, 1, 2)),   # 7-row groups of 34 pairs over 4 waves; tile 2 holds 2 rows
+    ("bcc", 1, (70, 9, 2), (7, 0, 2)),   # 7-row groups of 35 pairs, no XCD remap
+    ("c15", 3, (20, 7, 2), (6, 1, 2)),   # 6-row groups of 10 pairs in one wave
 ])
 def test_packed_rows_match_oracle(oracle, lattice_name, ncomp, dims, traversal):
     """x pairs spread over groups of 2-8 y rows (lines that one row per wave leaves partly idle)."""
@@ -213,6 +217,39 @@ def test_observables_match_oracle(oracle, lattice_name, ncomp, precision):
     assert o.finite
     if precision == 0:
         assert rel_l2(stored, psi) < 1e-7
+
+
+@pytest.mark.parametrize("lattice_name,ncomp,precision", [
+    ("bcc", 1, 1), ("bcc", 3, 1), ("fcc", 3, 1), ("a15", 5, 1), ("bcc", 3, 0), ("c15", 1, 0)])
+def test_max_hamiltonian_and_weighted_diff_match_oracle(oracle, lattice_name, ncomp, precision):
+    """gpe_max_hamiltonian (maxHamilton, Spin1GpeBCC.cu:144-189) and gpe_weighted_diff
+    (weightedDiff + integrate, :436-461,1620-1633) against the oracle on the stored state, with
+    a quadrupole field and a complex phase; the device-pointer form gives the same bits."""
+    import torch
+    from spacetime_amd import lattice
+    cfg = _config(lattice_name, ncomp, precision, (5, 3, 4))
+    L = oracle.lattice(lattice.get(lattice_name))
+    p = with_field(oracle_params(oracle, cfg), **FIELD)
+    psi = random_state(cfg.shape, 21)
+    ref = random_state(cfg.shape, 22)
+    ph = np.exp(-0.4j)
+    with gpe.Stepper(cfg) as st:
+        st.set_field(**FIELD)
+        st.set_state(psi)
+        stored = st.get_state(0)
+        if ncomp == 5:
+            with pytest.raises(gpe.GpeError):
+                st.max_hamiltonian()
+        else:
+            got = st.max_hamiltonian()
+            assert got == pytest.approx(oracle.max_hamiltonian(L, p, stored), rel=1e-13)
+        wd = st.weighted_diff(ref, ph)
+        dref = torch.from_numpy(np.ascontiguousarray(ref)).to("cuda")
+        wd_dev = st.weighted_diff(dref.data_ptr(), ph)
+        torch.cuda.synchronize()
+        assert st.weighted_diff(stored, 1.0) == 0.0
+    assert wd == wd_dev
+    assert wd == pytest.approx(oracle.weighted_diff(L, p, stored, ref, ph), rel=1e-12)
 
 
 def test_ramp_matches_oracle(oracle):

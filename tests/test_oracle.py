@@ -284,3 +284,71 @@ def test_ramp_segments(oracle):
     r2 = oracle.ramp(b_segments=[((0, 0, 0.0), 0.0, 0), ((0, 0, 2.0), 10.0, 1)], time_scale=1e3)
     _, bb = oracle.ramp_signal(r2, 2.5e-3)
     assert bb[2] == pytest.approx(0.5)
+
+
+# ------------------------------------------------------- Hamiltonian bound, weighted diff ---
+
+def test_max_hamiltonian_closed_forms(oracle):
+    """maxHamilton (Spin1GpeBCC.cu:144-189) on states where it has a closed form: the trap alone
+    (zero state: max V over the unknowns' positions), a uniform density (+ c0 n), z- and
+    x-polarised spin-1 states in a uniform bias (|B_a + c1 F_a|, 1/sqrt2 on x and y), and the
+    trap switched off."""
+    rng = np.random.default_rng(8)
+    dims = (3, 2, 4)
+    kw = dict(trap=(1.0, 1.3, 0.7), trap_offset=100.0, c0=7.0, c1=-3.0, bb=(0.2, -0.5, 0.9))
+    lat, L, p1 = _grid_params(oracle, "bcc", 1, dims, rng, **kw)
+    r = oracle.positions(L, p1, lat.V)
+    vmax = np.max(0.5 * np.sum((r * np.array(kw["trap"])) ** 2, -1)) + 100.0
+    shape1 = (1, dims[2], dims[1], dims[0], lat.V)
+    assert oracle.max_hamiltonian(L, p1, np.zeros(shape1, complex)) == pytest.approx(vmax, rel=1e-15)
+    a = 0.6
+    uni = np.full(shape1, a * np.exp(0.3j))
+    assert oracle.max_hamiltonian(L, p1, uni) == pytest.approx(vmax + 7.0 * a * a, rel=1e-14)
+    _, L3, p3 = _grid_params(oracle, "bcc", 3, dims, rng, **kw)
+    shape3 = (3,) + shape1[1:]
+    zpol = np.zeros(shape3, complex)
+    zpol[0] = a
+    pot = vmax + 7.0 * a * a
+    want = pot + max(0.2 / np.sqrt(2), 0.5 / np.sqrt(2), abs(0.9 - 3.0 * a * a))
+    assert oracle.max_hamiltonian(L3, p3, zpol) == pytest.approx(want, rel=1e-14)
+    xpol = np.zeros(shape3, complex)
+    xpol[:] = (a * np.array([0.5, 2 ** -0.5, 0.5]))[:, None, None, None, None]
+    want = pot + max(abs(0.2 - 3.0 * a * a) / np.sqrt(2), 0.5 / np.sqrt(2), 0.9)
+    assert oracle.max_hamiltonian(L3, p3, xpol) == pytest.approx(want, rel=1e-14)
+    _, L3, p3off = _grid_params(oracle, "bcc", 3, dims, rng, **dict(kw, trap_off_time=1.0))
+    assert oracle.max_hamiltonian(L3, p3off, zpol, t=2.0) == pytest.approx(
+        7.0 * a * a + max(0.2 / np.sqrt(2), 0.5 / np.sqrt(2), abs(0.9 - 3.0 * a * a)), rel=1e-14)
+    _, L5, p5 = _grid_params(oracle, "bcc", 5, dims, rng)
+    assert np.isnan(oracle.max_hamiltonian(L5, p5, np.zeros((5,) + shape1[1:], complex)))
+
+
+@pytest.mark.parametrize("lat_name,ncomp", [("bcc", 3), ("fcc", 5), ("a15", 1)])
+def test_weighted_diff_closed_forms(oracle, lat_name, ncomp):
+    """weightedDiff + integrate (Spin1GpeBCC.cu:436-461,1620-1633): zero at psi = a; psi = 0 on a
+    uniform reference gives n0 * norm(ref); a single populated unknown gives dv_v |a|^2 |psi-a|^2
+    (index and volume); phase covariance and degree-4 homogeneity."""
+    rng = np.random.default_rng(9)
+    dims = (2, 3, 2)
+    lat, L, p = _grid_params(oracle, lat_name, ncomp, dims, rng)
+    shape = (ncomp, dims[2], dims[1], dims[0], lat.V)
+    ref = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    psi = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    ph = np.exp(0.7j)
+    assert oracle.weighted_diff(L, p, ph * ref, ref, ph) == pytest.approx(0.0, abs=1e-12)
+    uni = np.zeros(shape, complex)
+    uni[:] = (rng.normal(size=ncomp) + 1j * rng.normal(size=ncomp))[:, None, None, None, None]
+    n0 = np.sum(np.abs(uni[:, 0, 0, 0, 0]) ** 2)
+    assert oracle.weighted_diff(L, p, np.zeros(shape, complex), uni) == pytest.approx(
+        n0 * oracle.norm(L, p, uni), rel=1e-13)
+    one = np.zeros(shape, complex)
+    at = (1, 2, 0, lat.V - 1)
+    one[(slice(None),) + at] = ref[(slice(None),) + at]
+    dv = p.block_scale ** 3 * lat.vol[lat.V - 1]
+    a2 = np.sum(np.abs(ref[(slice(None),) + at]) ** 2)
+    d2 = np.sum(np.abs(psi[(slice(None),) + at] - ref[(slice(None),) + at]) ** 2)
+    rest = np.sum(np.abs(psi) ** 2) - np.sum(np.abs(psi[(slice(None),) + at]) ** 2)
+    assert rest > 0  # psi elsewhere does not count where a = 0
+    assert oracle.weighted_diff(L, p, psi, one) == pytest.approx(dv * a2 * d2, rel=1e-13)
+    base = oracle.weighted_diff(L, p, psi, ref)
+    assert oracle.weighted_diff(L, p, ph * psi, ref, ph) == pytest.approx(base, rel=1e-13)
+    assert oracle.weighted_diff(L, p, 1.5 * psi, 1.5 * ref) == pytest.approx(1.5 ** 4 * base, rel=1e-13)

@@ -95,6 +95,10 @@ CASES = [  # lattice, ncomp, precision, dims
     ("bcc", 5, gpe.FP64, (70, 11, 4)), ("cube", 5, gpe.FP32, (33, 13, 6)), ("fcc", 3, gpe.FP32, (7, 17, 5)),
     ("a15", 1, gpe.FP32, (57, 10, 3)), ("c15", 5, gpe.FP32, (29, 6, 3)), ("z", 3, gpe.FP32, (66, 9, 4)),
     ("fcc", 1, gpe.FP32, (160, 5, 3)), ("cube", 3, gpe.FP32, (293, 4, 2)),
+    # row groups of 3, 6 and 7 rows (A15's 41 pairs, Z's 34, BCC's 35, C15's 10)
+    ("a15", 5, gpe.FP32, (82, 13, 2)), ("z", 5, gpe.FP32, (68, 15, 2)), ("bcc", 3, gpe.FP32, (70, 9, 2)),
+    ("c15", 1, gpe.FP32, (20, 7, 2)), ("a15", 5, gpe.FP32, (82, 7, 2)), ("z", 3, gpe.FP32, (68, 16, 2)),
+    ("bcc", 1, gpe.FP32, (70, 9, 2)), ("c15", 3, gpe.FP32, (20, 7, 2)),
 ]
 
 
@@ -105,7 +109,7 @@ def test_decode_stays_in_range_and_covers_the_box(lat, ncomp, precision, dims):
     xpls = (1, 2) if precision == gpe.FP32 else (1,)
     checked = 0
     for xpl in xpls:
-        for ty in (1, 2, 3, 4, 8, 16, 32, 64):
+        for ty in (1, 2, 3, 4, 5, 6, 7, 8, 12, 14, 16, 32, 64):
             for remap in (0, 1):
                 rc, p = _plan(cfg, ty, remap, xpl)
                 assert rc == 0
@@ -140,6 +144,23 @@ def test_decode_stays_in_range_and_covers_the_box(lat, ncomp, precision, dims):
                 assert count.min() == 1 and count.max() == 1, (ty, remap, xpl)
                 checked += 1
     assert checked >= 16
+
+
+@pytest.mark.parametrize("lat,ncomp,dims,ty,rows", [
+    ("a15", 5, (82, 13, 2), 12, 3),  # 41 pairs: 3 rows fill 96% of two waves
+    ("a15", 5, (82, 13, 2), 8, 1),   # 4 or 8 rows fill 85%: below the large-cell floor of 90%
+    ("z", 5, (68, 15, 2), 14, 7),    # 34 pairs: 7 rows fill 93% of four waves
+    ("z", 5, (68, 15, 2), 8, 1),
+    ("bcc", 3, (70, 9, 2), 7, 7),    # 35 pairs: 7 rows fill 96%
+    ("bcc", 3, (70, 9, 2), 8, 8),    # small cells take the best power of two (88%)
+    ("c15", 1, (20, 7, 2), 6, 6),    # 10 pairs: 6 rows fill 94% of one wave
+    ("c15", 5, (57, 9, 2), 8, 2),    # lines of <= 64 values: 2 rows
+])
+def test_row_group_choice(lat, ncomp, dims, ty, rows):
+    """The row group a tile height gets, and that the cases above cover groups of 3, 6 and 7."""
+    cfg = gpe.Config(lattice=lat, ncomp=ncomp, precision=gpe.FP32, dims=dims)
+    rc, p = _plan(cfg, ty, 1, 2)
+    assert rc == 0 and p.rows == rows and p.tile_rows == ty
 
 
 @pytest.mark.parametrize("nb", [1, 7, 8, 9, 63, 64, 1000, 4097])
