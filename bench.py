@@ -406,11 +406,13 @@ def run_adjoint(args, log):
         log("adjoint %s: J=%.9f, %.3f s forward+backward (forward %.1f ms, backward %.1f ms), checkpoints %.2f GB, "
             "segment %.2f GB" % (name, J, wall, res["forward_ms"], res["backward_ms"], res["checkpoint_bytes"] / 1e9,
                                   res["segment_bytes"] / 1e9))
-    # central differences of FP64 forward runs at h = 1e-4 and h / 2; their Richardson
-    # combination (4 D(h/2) - D(h)) / 3 removes the O(h^2) term, which for the trap frequencies
-    # is not small: a trap change dephases the condensate (phase ~ t omega r^2), so J has large
-    # higher derivatives next to its small first derivative near the overlap maximum
-    h = 1e-4
+    # central differences of FP64 forward runs at h and h / 2; their Richardson combination
+    # (4 D(h/2) - D(h)) / 3 removes the O(h^2) term, which for the trap frequencies is not small:
+    # a trap change dephases the condensate (phase ~ t omega r^2), so J has large higher
+    # derivatives next to its small first derivative near the overlap maximum.  h = 1e-4 for the
+    # trap frequencies, 1e-2 for the knots: their gradients (~1e-8) are so small that at 1e-4
+    # the rounding of J (~1e-16 / h) is already 1e-3 of them
+    h = np.array([1e-2] * 16 + [1e-4] * 3)
     r32 = out["fp32"]
     level_bytes = float(np.prod(cfg.dims) * cfg.V * 3 * 8)
     line = {
@@ -422,7 +424,7 @@ def run_adjoint(args, log):
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": "adjoint_spin1_bcc_64^3", "cells": list(cells), "box": box, "sweeps": nsweeps,
                    "checkpoint_every": K, "parameters": "16 Bz knots + 3 trap frequencies", "objective": "overlap",
-                   "fd_step": h, "fd_scheme": "central at h and h/2, Richardson-combined"},
+                   "fd_step": {"knots": 1e-2, "trap": 1e-4}, "fd_scheme": "central at h and h/2, Richardson-combined"},
         "memory_bytes": {"levels": 2 * level_bytes, "checkpoints": r32["checkpoint_bytes"],
                          "segment": r32["segment_bytes"]},
         "forward_ms": r32["forward_ms"], "backward_ms": r32["backward_ms"],
@@ -439,9 +441,10 @@ def run_adjoint(args, log):
     fd = RampTrapProblem(cfg64, psi0, nsweeps, K, duration, objective=gpe.OBJ_OVERLAP, target=target, bq=4e-3)
     t0 = time.perf_counter()
 
-    def central(step):
+    def central(steps):
         g = np.zeros(19)
         for j in range(19):
+            step = steps[j]
             kp, km = knots.copy(), knots.copy()
             tp, tm = np.array(cfg.trap, float), np.array(cfg.trap, float)
             if j < 16:
@@ -466,15 +469,15 @@ def run_adjoint(args, log):
         g = np.concatenate([r["grad_knots"], r["grad_trap"]])
         err[name] = {k: float(np.abs(g[sl] - g_fd[sl]).max() / np.abs(g_fd[sl]).max()) for k, sl in groups.items()}
         err_raw[name] = {k: float(np.abs(g[sl] - g_h[sl]).max() / np.abs(g_h[sl]).max()) for k, sl in groups.items()}
-        log("%s: max |adjoint - FD| / max |FD|: knots %.2e, trap %.2e (plain h = 1e-4: %.2e, %.2e)" % (
+        log("%s: max |adjoint - FD| / max |FD|: knots %.2e, trap %.2e (plain h: %.2e, %.2e)" % (
             name, err[name]["knots"], err[name]["trap"], err_raw[name]["knots"], err_raw[name]["trap"]))
     line["accuracy"].update({
         "max_rel_grad_err_fp32_vs_fd": max(err["fp32"].values()),
         "max_rel_grad_err_fp32_fp64_trajectory_vs_fd": max(err["fp32_fp64_trajectory"].values()),
         "max_rel_grad_err_fp64_vs_fd": max(err["fp64"].values()),
-        "per_group": err, "per_group_vs_plain_fd_h1e-4": err_raw,
+        "per_group": err, "per_group_vs_plain_fd_h": err_raw,
         "grad": {k: np.concatenate([r["grad_knots"], r["grad_trap"]]).tolist() for k, r in out.items()},
-        "grad_fd_richardson": g_fd.tolist(), "grad_fd_h1e-4": g_h.tolist()})
+        "grad_fd_richardson": g_fd.tolist(), "grad_fd_h": g_h.tolist()})
     return line
 
 

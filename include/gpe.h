@@ -318,7 +318,7 @@ typedef struct {
 /* Precision limit.  With FP32 storage of psi and mu each sweep rounds both to 6e-8; gradients
  * that are small residues of cancelling per-sweep terms do not survive it.  Measured (spin-1
  * BCC, c0 1e4, 4000 sweeps, overlap near its maximum): the trap-frequency gradients (~6e-7,
- * per-sweep terms ~1e-2) come out 14-16% off from an FP32 context against central
+ * per-sweep terms ~1e-2) come out 14-17% off from an FP32 context against central
  * differences, and on the FP64 oracle rounding only the stored trajectory to FP32 moves them
  * by 19-47%, rounding only mu by up to 2.7%; the ramp-knot gradients stay within 3e-5 either
  * way.  fp64_trajectory = 1 gives the FP64 result (within 1e-4 there) on an FP32 context. */

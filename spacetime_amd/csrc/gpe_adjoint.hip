@@ -224,6 +224,41 @@ __device__ __forceinline__ void adj_sens(const R *pr, const R *pi, const R *mr, 
     }
 }
 
+// The same sensitivities for the FP32 pair body, formed in FP32 per unknown and summed over the
+// lane's two x before they join the FP64 partials (one conversion per parameter and lane instead
+// of one per product); the trap terms leave out the frequency factor omega_a, applied in FP64 when
+// the lane's sums are added (they are linear in it).
+template <int C>
+__device__ __forceinline__ void adj_sens32(const float *pr, const float *pi, const float *mr, const float *mi,
+                                           float rx, float ry, float rz, const AdjConsts<float> &k, bool trap_on,
+                                           float *s)
+{
+    float im_mp = 0.f;
+#pragma unroll
+    for (int c = 0; c < C; ++c) im_mp += mr[c] * pi[c] - mi[c] * pr[c];
+    if (trap_on) {
+        s[ADJ_WX] += rx * rx * im_mp;
+        s[ADJ_WY] += ry * ry * im_mp;
+        s[ADJ_WZ] += rz * rz * im_mp;
+    }
+    if constexpr (C > 1) {
+        const float sg = (C == 3) ? 1.f : -1.f;
+        float tr[C], ti[C], pb[3];
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            spin_dot<float, C>(ax == 0 ? 1.f : 0.f, ax == 1 ? 1.f : 0.f, ax == 2 ? 1.f : 0.f, pr, pi, tr, ti);
+            float im = 0.f;
+#pragma unroll
+            for (int c = 0; c < C; ++c) im += mr[c] * ti[c] - mi[c] * tr[c];
+            pb[ax] = sg * im;
+        }
+        s[ADJ_BX] += pb[0];
+        s[ADJ_BY] += pb[1];
+        s[ADJ_BZ] += pb[2];
+        s[ADJ_BQ] += (rx - k.center[0]) * pb[0] + (ry - k.center[1]) * pb[1] - 2.f * (rz - k.center[2]) * pb[2];
+    }
+}
+
 // One x value per lane (FP64, and FP32 where the pair form is not chosen): the forward sweep's
 // single body on mu_{k+1} -- interior fast path through the per-face line offsets, wave-uniform
 // weights on same-x faces, zero-weight pad faces skipped.
@@ -357,10 +392,12 @@ typedef float f4u __attribute__((ext_vector_type(4), aligned(8)));
 
 // Two consecutive x values per lane (FP32), optionally spread over a group of y rows: the
 // forward sweep's pair body (gpe_kernels.hip, sweep_body_pair) on mu_{k+1}, with psi_k and
-// mu_{k+2} read as own pairs.  A neighbour pair starts at x0 + dx, clamped into the line and
-// shifted back; every neighbour value outside the box is replaced by zero with a select, so line
-// padding is never used as a value.  The lone last x of an odd-length line writes its partner
-// slot, which is line padding.
+// mu_{k+2} read as own pairs, mu_{k+2} only after G'^T mu is formed (136 -> 110 VGPRs for spin-1:
+// 4 waves per SIMD instead of 3; the forward sweep, whose psi_{k-1} could be read as late, ran
+// 2-20% slower that way and keeps its early load).  A neighbour pair starts at x0 + dx, clamped
+// into the line and shifted back; every neighbour value outside the box is replaced by zero with
+// a select, so line padding is never used as a value.  The lone last x of an odd-length line
+// writes its partner slot, which is line padding.
 template <int C, int DEG, bool ROWS>
 __device__ __forceinline__ void adjoint_pair(const AdjArgs &a, int b, int nb, double *part)
 {
@@ -395,12 +432,11 @@ __device__ __forceinline__ void adjoint_pair(const AdjArgs &a, int b, int nb, do
     const long long own_line = line_index(g, 0, z, y0, v);
     const int lxm2 = (int)g.lx - 2;
 
-    float pr[2][C], pi[2][C], mr[2][C], mi[2][C], qr[2][C], qi[2][C], ar[2][C], ai[2][C];
+    float pr[2][C], pi[2][C], mr[2][C], mi[2][C], ar[2][C], ai[2][C];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         const f4a p = *(const f4a *)(psi + own_line + c * cstride + lo);
         const f4a m = *(const f4a *)(mu1 + own_line + c * cstride + lo);
-        const f4a q = __builtin_nontemporal_load((const f4a *)(mu2 + own_line + c * cstride + lo));
         pr[0][c] = p.x;
         pi[0][c] = p.y;
         pr[1][c] = p.z;
@@ -409,10 +445,6 @@ __device__ __forceinline__ void adjoint_pair(const AdjArgs &a, int b, int nb, do
         mi[0][c] = m.y;
         mr[1][c] = m.z;
         mi[1][c] = m.w;
-        qr[0][c] = q.x;
-        qi[0][c] = q.y;
-        qr[1][c] = q.z;
-        qi[1][c] = q.w;
         ar[0][c] = ar[1][c] = 0.f;
         ai[0][c] = ai[1][c] = 0.f;
     }
@@ -523,6 +555,9 @@ __device__ __forceinline__ void adjoint_pair(const AdjArgs &a, int b, int nb, do
     const float rz = k.p0[2] + (k.cell[2] * (float)z + pos[2]);
     const float coef = (float)a.coef;
     float2 out[2][C];
+    float s32[ADJ_COUNT];
+#pragma unroll
+    for (int i = 0; i < ADJ_COUNT; ++i) s32[i] = 0.f;
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
         float hr[C], hi[C], orr[C], oi[C];
@@ -535,11 +570,27 @@ __device__ __forceinline__ void adjoint_pair(const AdjArgs &a, int b, int nb, do
         adj_point<float, C>(pr[e], pi[e], mr[e], mi[e], hr, hi, rx, ry, rz, trap_on, a.quad, k, orr, oi);
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-            out[e][c].x = qr[e][c] + coef * orr[c];
-            out[e][c].y = qi[e][c] + coef * oi[c];
+            out[e][c].x = coef * orr[c];
+            out[e][c].y = coef * oi[c];
         }
         if (e == 0 ? act0 : act1)
-            adj_sens<float, C>(pr[e], pi[e], mr[e], mi[e], a.ph, a.tb.pos64 + 3 * v, x0 + e, y, z, trap_on, part);
+            adj_sens32<C>(pr[e], pi[e], mr[e], mi[e], rx, ry, rz, k, trap_on, s32);
+    }
+    part[ADJ_BX] += (double)s32[ADJ_BX];
+    part[ADJ_BY] += (double)s32[ADJ_BY];
+    part[ADJ_BZ] += (double)s32[ADJ_BZ];
+    part[ADJ_BQ] += (double)s32[ADJ_BQ];
+    part[ADJ_WX] += a.ph.trap[0] * (double)s32[ADJ_WX];
+    part[ADJ_WY] += a.ph.trap[1] * (double)s32[ADJ_WY];
+    part[ADJ_WZ] += a.ph.trap[2] * (double)s32[ADJ_WZ];
+    // mu_{k+2} is read only now: its registers are free while G'^T mu is formed
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const f4a q = __builtin_nontemporal_load((const f4a *)(mu2 + own_line + c * cstride + lo));
+        out[0][c].x += q.x;
+        out[0][c].y += q.y;
+        out[1][c].x += q.z;
+        out[1][c].y += q.w;
     }
     if (act0) {
 #pragma unroll

@@ -224,8 +224,10 @@ def test_observables_match_oracle(oracle, lattice_name, ncomp, precision):
 def test_max_hamiltonian_and_weighted_diff_match_oracle(oracle, lattice_name, ncomp, precision):
     """gpe_max_hamiltonian (maxHamilton, Spin1GpeBCC.cu:144-189) and gpe_weighted_diff
     (weightedDiff + integrate, :436-461,1620-1633) against the oracle on the stored state, with
-    a quadrupole field and a complex phase; the device-pointer form gives the same bits."""
-    import torch
+    a quadrupole field and a complex phase; the device-pointer form gives the same bits.  The
+    device copy of the reference is made with the HIP runtime libgpe itself links (torch ships
+    its own, which cannot share the device with it once libgpe has initialised HIP)."""
+    import ctypes
     from spacetime_amd import lattice
     cfg = _config(lattice_name, ncomp, precision, (5, 3, 4))
     L = oracle.lattice(lattice.get(lattice_name))
@@ -244,9 +246,18 @@ def test_max_hamiltonian_and_weighted_diff_match_oracle(oracle, lattice_name, nc
             got = st.max_hamiltonian()
             assert got == pytest.approx(oracle.max_hamiltonian(L, p, stored), rel=1e-13)
         wd = st.weighted_diff(ref, ph)
-        dref = torch.from_numpy(np.ascontiguousarray(ref)).to("cuda")
-        wd_dev = st.weighted_diff(dref.data_ptr(), ph)
-        torch.cuda.synchronize()
+        hip = gpe.load_library()  # dlsym through libgpe: its own HIP runtime
+        hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        hip.hipFree.argtypes = [ctypes.c_void_p]
+        host = np.ascontiguousarray(ref, dtype=np.complex128)
+        dref = ctypes.c_void_p()
+        assert hip.hipMalloc(ctypes.byref(dref), host.nbytes) == 0
+        try:
+            assert hip.hipMemcpy(dref, host.ctypes.data, host.nbytes, 1) == 0  # host to device
+            wd_dev = st.weighted_diff(dref.value, ph)
+        finally:
+            hip.hipFree(dref)
         assert st.weighted_diff(stored, 1.0) == 0.0
     assert wd == wd_dev
     assert wd == pytest.approx(oracle.weighted_diff(L, p, stored, ref, ph), rel=1e-12)
