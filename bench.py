@@ -278,10 +278,15 @@ def run_lattices(args, log):
     target = 25165824
     cases = []
     tot_nodes_steps, tot_s, tot_bytes, tot_kernel_s = 0.0, 0.0, 0.0, 0.0
+    only = None
+    if args.cases:  # "lattice:C,lattice:C": a subset of the cases (profiling runs)
+        only = {(c.split(":")[0], int(c.split(":")[1])) for c in args.cases.split(",")}
     for name in ("cube", "bcc", "fcc", "a15", "c15", "z"):
         lat = lattice.get(name)
         n = int(round((target / lat.V) ** (1.0 / 3.0)))
         for ncomp in (1, 3, 5):
+            if only is not None and (name, ncomp) not in only:
+                continue
             kw = dict(c0=gmu)
             if ncomp == 3:
                 kw.update(c1=500.0, q=0.1)
@@ -308,7 +313,11 @@ def run_lattices(args, log):
                     st.set_state(psi0)
                     o0 = st.observables(0)
                     st.start(gpe.START_EULER)
-                    tune = st.autotune(args.tune_sweeps) if (timed and args.tune_sweeps) else st.traversal()
+                    if timed and args.traversal:
+                        st.set_traversal(*args.traversal)
+                        tune = st.traversal()
+                    else:
+                        tune = st.autotune(args.tune_sweeps) if (timed and args.tune_sweeps) else st.traversal()
                     st.step(args.warmup)
                     st.sync()
                     t0 = time.perf_counter()
@@ -662,6 +671,8 @@ def main():
                     help="autotuner trial sweeps per traversal candidate (0: default traversal)")
     ap.add_argument("--traversal", type=lambda v: tuple(int(x) for x in v.split(",")), default=None,
                     help="fixed FP32 sweep traversal 'rows,remap,x_per_lane' instead of the autotuner")
+    ap.add_argument("--cases", default=None,
+                    help="lattices workload: only these cases, 'lattice:C,...' (e.g. 'a15:5,bcc:1')")
     ap.add_argument("--itp-steps", type=int, default=1000)
     ap.add_argument("--itp-dt", type=float, default=2e-4)
     ap.add_argument("--no-cpu", action="store_true")

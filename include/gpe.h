@@ -226,8 +226,9 @@ int gpe_ensemble_step(gpe_ensemble *ens, int nsteps, int graph_sweeps);
 int gpe_ensemble_timing(const gpe_ensemble *ens, double *ms_per_sweep, double *unknowns);
 
 /* -------------------------------------------------------------------- autotuning ---- */
-/* Time the traversal candidates (y-tile rows 1..32 x XCD remap on/off x one or two x values
- * per lane, the latter for FP32) on scratch copies of the context's
+/* Time the traversal candidates (y-tile rows 1..32, and 3..28 where they give row groups that
+ * are not powers of two, x XCD remap on/off x one or two x values per lane, the latter for
+ * FP32) on scratch copies of the context's
  * two levels for trial_sweeps sweeps each, and keep the fastest.  Results are cached per
  * (lattice, ncomp, precision, nx, ny, nz) for the process; pass trial_sweeps = 0 to apply a
  * cached choice only.  Outputs may be NULL.  No counterpart in the reference, which fixes its

@@ -116,6 +116,8 @@ def test_traversal_setter_validates():
             st.set_traversal(8, 1, 2)  # FP64 has one x per lane only
         with pytest.raises(gpe.GpeError, match="EINVAL"):
             st.set_traversal(0, 1, 1)
+        with pytest.raises(gpe.GpeError, match="EINVAL"):
+            st.set_traversal(8, 1, 4)  # one or two x per lane only
         st.set_traversal(4, 0, 1)
         assert st.traversal() == dict(tile_rows=4, xcd_remap=0, x_per_lane=1)
 
@@ -169,12 +171,13 @@ def test_itp_matches_oracle(oracle, ncomp):
     assert rel_l2(got, ref) < 1e-12
 
 
-@pytest.mark.parametrize("dims,traversal", [((4, 4, 3), None), ((6, 7, 3), (4, 1, 2))])
-def test_fp32_itp_matches_oracle(oracle, dims, traversal):
+@pytest.mark.parametrize("dims,traversal,ncomp", [((4, 4, 3), None, 3), ((6, 7, 3), (4, 1, 2), 3),
+                                                  ((6, 7, 3), (4, 1, 2), 1)])
+def test_fp32_itp_matches_oracle(oracle, dims, traversal, ncomp):
     """FP32 imaginary time (FP64 norm and renormalisation): the default traversal and x pairs over
-    4 rows per wave with rows past ny."""
+    4 rows per wave with rows past ny, three components and one."""
     from spacetime_amd import lattice
-    cfg = _config("bcc", 3, gpe.FP32, dims)
+    cfg = _config("bcc", ncomp, gpe.FP32, dims)
     L = oracle.lattice(lattice.get("bcc"))
     p = oracle_params(oracle, cfg)
     psi = random_state(cfg.shape, 9)
@@ -401,12 +404,15 @@ def test_expanding_grid_matches_oracle(oracle, precision, tol):
     assert rel_l2(got, b) < tol
 
 
-@pytest.mark.parametrize("xpl,nx", [(1, 7), (2, 7), (2, 66), (1, 66)])
+@pytest.mark.parametrize("xpl,nx", [(1, 7), (2, 7), (2, 66), (1, 66), (2, 300)])
 def test_fp32_staggered_matches_oracle(oracle, xpl, nx):
     """The FP32 form of the scalar staggered scheme (odd += -i dt H(even); even += -i dt H(odd),
-    GPBlocks.cpp:560-606) with both lane widths, odd and even line lengths, phase start."""
+    GPBlocks.cpp:560-606) with one and two x per lane, odd and even line lengths, phase start;
+    the 300-block box (two waves per line) has a weaker x trap so that dt V stays below the
+    scheme's limit."""
     from spacetime_amd import lattice
-    cfg = _config("bcc", 1, gpe.FP32, (nx, 5, 4), dt=1e-3, scheme=gpe.STAGGERED, c1=0.0, c2=0.0, q=0.0)
+    trap = (1.0 if nx <= 100 else 30.0 / (0.6 * nx), 0.9, 1.2)
+    cfg = _config("bcc", 1, gpe.FP32, (nx, 5, 4), dt=1e-3, scheme=gpe.STAGGERED, c1=0.0, c2=0.0, q=0.0, trap=trap)
     L = oracle.lattice(lattice.get("bcc"))
     p = oracle_params(oracle, cfg)
     even = random_state(cfg.shape, 6)

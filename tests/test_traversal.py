@@ -99,6 +99,8 @@ CASES = [  # lattice, ncomp, precision, dims
     ("a15", 5, gpe.FP32, (82, 13, 2)), ("z", 5, gpe.FP32, (68, 15, 2)), ("bcc", 3, gpe.FP32, (70, 9, 2)),
     ("c15", 1, gpe.FP32, (20, 7, 2)), ("a15", 5, gpe.FP32, (82, 7, 2)), ("z", 3, gpe.FP32, (68, 16, 2)),
     ("bcc", 1, gpe.FP32, (70, 9, 2)), ("c15", 3, gpe.FP32, (20, 7, 2)),
+    # one component over long lines: several waves per line
+    ("cube", 1, gpe.FP32, (293, 4, 2)), ("bcc", 1, gpe.FP32, (600, 3, 2)), ("z", 1, gpe.FP32, (131, 5, 2)),
 ]
 
 
@@ -180,5 +182,6 @@ def test_workgroup_orders_are_permutations(nb):
 def test_plan_rejects_bad_arguments():
     cfg = gpe.Config(lattice="bcc", ncomp=3, precision=gpe.FP64, dims=(8, 8, 8))
     assert _plan(cfg, 8, 1, 2)[0] == -22  # FP64 has one x per lane
+    assert _plan(gpe.Config(lattice="bcc", ncomp=1, dims=(8, 8, 8)), 8, 1, 4)[0] == -22
     assert _plan(cfg, 0, 1, 1)[0] == -22
     assert _plan(gpe.Config(lattice="bcc", dims=(0, 8, 8)), 8, 1, 1)[0] == -22
