@@ -445,6 +445,14 @@ def run_adjoint(args, log):
         "accuracy": {"J_" + k: r["J"] for k, r in out.items()},
         "roofline": None, "cpu_baseline": None,
     }
+    # agreement of the FP32 forms with the FP64 adjoint, per group (relative to the group's largest
+    # FP64 gradient): needs no finite differences
+    g64 = np.concatenate([out["fp64"]["grad_knots"], out["fp64"]["grad_trap"]])
+    grp = {"knots": slice(0, 16), "trap": slice(16, 19)}
+    line["accuracy"]["rel_diff_vs_fp64_adjoint"] = {
+        k: {gn: float(np.abs(np.concatenate([r["grad_knots"], r["grad_trap"]])[sl] - g64[sl]).max() /
+                      np.abs(g64[sl]).max()) for gn, sl in grp.items()}
+        for k, r in out.items() if k != "fp64"}
     if args.no_fd:  # timing only (profiling runs): no finite-difference check
         return line
     fd = RampTrapProblem(cfg64, psi0, nsweeps, K, duration, objective=gpe.OBJ_OVERLAP, target=target, bq=4e-3)
@@ -678,6 +686,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-fd", action="store_true", help="adjoint workload: skip the finite-difference check")
+    ap.add_argument("--no-other-configs", action="store_true",
+                    help="default workload: skip the short runs of the scalar, adjoint and ensemble configurations")
     ap.add_argument("--quiet", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -773,7 +783,49 @@ def main():
             "achieved_gbs": r64["achieved_gbs"], "frac": r64["achieved_gbs"] / PEAK_HBM_GBS},
         "accuracy": acc,
     }
+    if args.workload == "spin1" and not args.no_other_configs:
+        line["other_configs"] = run_other_configs(log)
     print(json.dumps(line))
+
+
+OTHER_CONFIGS = (
+    # name, arguments of a child bench run, seconds allowed
+    ("scalar_bcc_192^3", ["--workload", "scalar", "--steps", "1000", "--warmup", "20", "--no-cpu"], 400),
+    ("adjoint_spin1_bcc_64^3", ["--workload", "adjoint", "--no-fd"], 400),
+    ("ensemble_spin2_48", ["--workload", "ensemble", "--steps", "100", "--graph-sweeps", "20"], 600),
+)
+
+
+def run_other_configs(log):
+    """The other configurations' lines, each from a child `bench.py` run (so that a failure there
+    cannot take the headline line with it), in short forms: the scalar BCC 192^3 staggered run
+    for 1000 steps, the spin-1 BCC 64^3 adjoint without its finite-difference check (the FP32
+    forms' gradients against the FP64 adjoint's instead), the 48-member spin-2 ensemble for 100
+    steps.  The full-length runs are `--workload scalar|adjoint|ensemble`."""
+    import subprocess
+    out = {}
+    for name, argv, limit in OTHER_CONFIGS:
+        t0 = time.perf_counter()
+        log("other configuration %s: bench.py %s" % (name, " ".join(argv)))
+        try:
+            r = subprocess.run([sys.executable, os.path.abspath(__file__)] + argv, capture_output=True, text=True,
+                               timeout=limit)
+            lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+            if r.returncode != 0 or not lines:
+                tail = (r.stderr or "").strip().splitlines()[-3:]
+                out[name] = {"error": "exit %d: %s" % (r.returncode, " | ".join(tail))}
+                continue
+            res = json.loads(lines[-1])
+            keep = {k: res.get(k) for k in ("metric", "value", "unit", "steps", "ms_per_step", "dtype", "config")}
+            for k in ("roofline", "accuracy", "fp64_form", "fp32_fp64_trajectory_form", "memory_bytes",
+                      "forward_ms", "backward_ms", "small_members", "speedup_vs_sequential"):
+                if k in res:
+                    keep[k] = res[k]
+            keep["wall_s"] = time.perf_counter() - t0
+            out[name] = keep
+        except subprocess.TimeoutExpired:
+            out[name] = {"error": "timed out after %d s" % limit}
+    return out
 
 
 if __name__ == "__main__":
