@@ -54,5 +54,20 @@ def build(verbose=False):
     return LIB
 
 
+def build_driver(verbose=False):
+    """Compile the C++ host driver of the tests (tests/cxx/gpe_driver.cpp: the C ABI used from C++
+    alone) against include/ and link it to the in-tree libgpe.so; returns the binary's path."""
+    src = os.path.join(ROOT, "tests", "cxx", "gpe_driver.cpp")
+    out = os.path.join(ROOT, "tests", "cxx", "gpe_driver")
+    cxx = shutil.which("g++") or "g++"
+    cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-I" + INCLUDE, src, "-L" + HERE, "-lgpe",
+           "-Wl,-rpath,$ORIGIN/../../spacetime_amd", "-o", out]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.run(cmd, check=True)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_driver(verbose="-v" in sys.argv))

@@ -73,3 +73,37 @@ def test_create_rejects_line_counts_past_32_bits_without_device(lib):
     cfg = gpe.Config(lattice="bcc", dims=(2, 13400, 13400), ncomp=1, precision=gpe.FP32).to_c()
     assert lib.gpe_create(ctypes.byref(cfg), ctypes.byref(h)) == -22
     assert not h.value
+
+
+def test_header_compiles_as_c_and_every_function_links(tmp_path, lib):
+    """include/gpe.h is plain C: a C99 file that takes the address of every declared function
+    compiles and links against libgpe.so (the C linkage a cgo / ctypes / JNI binding needs)."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    names = declared_functions()
+    src = tmp_path / "abi_check.c"
+    src.write_text('#include "gpe.h"\n#include <stdio.h>\nint main(void) {\n    void *f[] = {\n' +
+                   "".join("        (void *)%s,\n" % n for n in names) +
+                   '    };\n    printf("%d\\n", (int)(sizeof f / sizeof f[0]));\n    return 0;\n}\n')
+    out = tmp_path / "abi_check"
+    subprocess.run([cc, "-std=c99", "-Wall", "-Werror", "-pedantic", "-Wno-pedantic", "-I" + os.path.join(ROOT, "include"),
+                    str(src), "-L" + os.path.dirname(LIB), "-lgpe", "-o", str(out)], check=True)
+    assert len(names) >= 58
+
+
+def test_cxx_driver_compiles_and_links(tmp_path, lib):
+    """The C++ host driver (tests/cxx/gpe_driver.cpp) builds against the header and the library
+    alone; the GPU test runs the copy that the build step makes."""
+    import shutil
+    import subprocess
+    cxx = shutil.which("g++")
+    if cxx is None:
+        pytest.skip("no C++ compiler")
+    out = tmp_path / "gpe_driver"
+    subprocess.run([cxx, "-std=c++17", "-Wall", "-Werror", "-I" + os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cxx", "gpe_driver.cpp"), "-L" + os.path.dirname(LIB), "-lgpe",
+                    "-o", str(out)], check=True)
+    assert out.exists()
