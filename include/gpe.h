@@ -439,6 +439,34 @@ int gpe_host_write_vtk_preimage(const gpe_config *cfg, const double *psi, const 
                                 double threshold);
 int gpe_host_write_vtk_volume(const gpe_config *cfg, const double *psi, const char *path, int basis);
 
+/* ------------------------------------------------------------------ BMP pictures ---- */
+/* The reference's density pictures, byte for byte (Picture::save, Output/Picture.cpp:83-138:
+ * 24-bit BMP, rows from y = 0 up, channel value v stored as 256 v truncated and clamped to
+ * 0..255; the header's file and image sizes count 4 bytes per pixel as the reference's do).
+ * With W = 2 (nx + 2), H = 2 (ny + 2), D = 2 (nz + 2) pixels along the padded grid (two pixels
+ * per block edge, the zero halo included):
+ *   GPE_PICTURE_DENSITY  spinor states: per component m, the density |psi_m|^2 in the basis
+ *       `basis` summed along one axis, grey, scaled by 1 / the plane's largest sum.
+ *       spin-1 (drawDensity, Spin1GpeBCC/utils.h:151-366): 3 W x 2 H, XZ (y < D) and XY
+ *       (from y = H), both mirrored in x, XY also in y.  spin-2 (drawDensity,
+ *       Spin2GpeBCC/utils.h:287-796): 5 W x 3 H, XZ, YZ (scaled by the XZ maximum, as there),
+ *       XY.  Grey 0.5 separator lines between the panels.
+ *   GPE_PICTURE_RI  spinor states: red = sum of Re^2, green = sum of Im^2.  spin-1
+ *       (drawDensityRI, Spin1GpeBCC/utils.h:368-580): 3 W x 2 H, XZ and XY, times 2 / the
+ *       plane's largest density sum; spin-2 (drawIandR, Spin2GpeBCC/utils.h:131-285): 5 W x 2 H,
+ *       unscaled, z basis only.
+ *   GPE_PICTURE_SLICE  scalar states (GPUGrossPitaevskii_BCC.cu:700-723): W x H, the padded
+ *       block layer nz / 2 + 1, unknown 0: red = 20 Re^2, green = 20 Im^2.
+ * The reference names its pictures dir + "/" + name + t + "ms.bmp" with t in fixed notation
+ * and 18 decimals (the checkpoint tag's format); path is the whole file name here.
+ * GPE_EINVAL for a style the state's component count has no picture for, or a basis other
+ * than z for a spin-2 GPE_PICTURE_RI picture. */
+#define GPE_PICTURE_DENSITY 0
+#define GPE_PICTURE_RI 1
+#define GPE_PICTURE_SLICE 2
+int gpe_write_bmp(gpe_ctx *ctx, int level, const char *path, int style, int basis);
+int gpe_host_write_bmp(const gpe_config *cfg, const double *psi, const char *path, int style, int basis);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,11 +1,12 @@
 // Host-side files of libgpe: the reference solvers' state files (.dat) and their checkpoint
-// pair, and the legacy VTK files their viewers read.  Everything is written from (or read into)
+// pair, the legacy VTK files their viewers read, and their BMP density pictures.  Everything is written from (or read into)
 // a level of a context through the ABI state layout psi[c][z][y][x][v] (gpe_get_state /
 // gpe_set_state); the formats are the reference's, byte for byte where its arithmetic is
 // correctly rounded (sqrt, division), so files move between the two codes.
 #include "gpe.h"
 #include "lattice_tables.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -448,6 +449,260 @@ int write_volume(const Level &L, const char *path, int basis)
     return o.close() ? GPE_OK : GPE_EDEVICE;
 }
 
+// ---------------------------------------------------------------------- BMP pictures --
+
+// Spin-1 components in the quantisation basis of the reference's BASIS switch
+// (Spin1GpeBCC/utils.h:176-190): s' = U s with the rotation of the m = 1, 0, -1 basis onto the
+// x or y axis, evaluated as the reference writes it (halves, then the sqrt 2 divisions;
+// s0 * (-i) = (im, -re) exactly).
+void spin1_basis(int basis, double *re, double *im)
+{
+    if (basis == GPE_BASIS_Z) return;
+    const double s2 = std::sqrt(2.0);
+    const double r1 = re[0], r0 = re[1], rm = re[2];
+    const double i1 = im[0], i0 = im[1], imm = im[2];
+    if (basis == GPE_BASIS_X) {
+        re[0] = 0.5 * (r1 + rm) - r0 / s2;
+        im[0] = 0.5 * (i1 + imm) - i0 / s2;
+        re[1] = (r1 - rm) / s2;
+        im[1] = (i1 - imm) / s2;
+        re[2] = 0.5 * (rm + r1) + r0 / s2;
+        im[2] = 0.5 * (imm + i1) + i0 / s2;
+    } else {
+        re[0] = 0.5 * (r1 - rm) + i0 / s2;
+        im[0] = 0.5 * (i1 - imm) + (-r0) / s2;
+        re[1] = (i1 + imm) / s2;
+        im[1] = (-(r1 + rm)) / s2;
+        re[2] = 0.5 * (rm - r1) + i0 / s2;
+        im[2] = 0.5 * (imm - i1) + (-r0) / s2;
+    }
+}
+
+// An RGB picture saved as the reference's Picture::save writes it (Output/Picture.cpp:83-138):
+// "BM", a 40-byte info header, 24-bit pixels as blue, green, red bytes, rows from y = 0 up, each
+// padded to 4 bytes; a channel value v is stored as 256 v truncated, 0 below 1 and 255 above
+// 255.  The header's file size and image size count 4 bytes per pixel, as the reference's do,
+// so the files match it byte for byte.  Pixels start black.
+class Picture {
+public:
+    Picture(int w, int h) : w_(w), h_(h), px_((size_t)w * h * 3, 0.0) {}
+    int width() const { return w_; }
+    int height() const { return h_; }
+    // Writes outside the picture are dropped (the reference's setColor drops x >= width).
+    void set(int x, int y, double r, double g, double b)
+    {
+        if (x < 0 || y < 0 || x >= w_ || y >= h_) return;
+        double *p = &px_[3 * ((size_t)y * w_ + x)];
+        p[0] = r;
+        p[1] = g;
+        p[2] = b;
+    }
+    int save(const char *path) const
+    {
+        Out o(path);
+        if (!o.ok()) return GPE_EINVAL;
+        const uint32_t offs = 14 + 40, data = 4u * (uint32_t)((size_t)w_ * h_), size = offs + data;
+        const uint32_t zero = 0, info = 40, w = (uint32_t)w_, h = (uint32_t)h_;
+        const uint16_t planes = 1, bits = 24;
+        o.bytes("BM", 2);
+        o.bytes(&size, 4);
+        o.bytes(&zero, 4);
+        o.bytes(&offs, 4);
+        o.bytes(&info, 4);
+        o.bytes(&w, 4);
+        o.bytes(&h, 4);
+        o.bytes(&planes, 2);
+        o.bytes(&bits, 2);
+        o.bytes(&zero, 4);
+        o.bytes(&data, 4);
+        for (int k = 0; k < 4; ++k) o.bytes(&zero, 4);
+        const int pad = 3 - ((3 * w_ + 3) % 4);
+        std::vector<unsigned char> row((size_t)3 * w_ + pad, 0);
+        for (int y = 0; y < h_; ++y) {
+            const double *p = &px_[3 * (size_t)y * w_];
+            for (int x = 0; x < w_; ++x) {
+                row[3 * x + 0] = channel(p[3 * x + 2]);
+                row[3 * x + 1] = channel(p[3 * x + 1]);
+                row[3 * x + 2] = channel(p[3 * x + 0]);
+            }
+            o.bytes(row.data(), row.size());
+        }
+        return o.close() ? GPE_OK : GPE_EDEVICE;
+    }
+
+private:
+    static unsigned char channel(double v)
+    {
+        const double c = 256.0 * v;
+        if (!(c >= 1.0)) return 0;  // also NaN, which the reference's conversion turns into 0
+        if (c > 255.0) return 255;
+        return (unsigned char)(int)c;
+    }
+    int w_, h_;
+    std::vector<double> px_;
+};
+
+// Densities of the reference's pictures projected onto a plane of the padded block grid: for
+// block (a, b) of the plane's two axes and component m, the sums over the third axis of
+// |psi_m|^2, of Re^2 and of Im^2 (in the picture's basis).  The reference sums pixel by pixel
+// with two pixels per block edge, so every block of the column enters twice in a row, each
+// time with its unknowns v = 0..V-1 in turn; the sums keep that order.  Halo blocks hold
+// zeros, and skipping them leaves every sum unchanged.
+struct Plane {
+    int na = 0, nb = 0, C = 0;
+    std::vector<double> n, r, i;  // [(b * na + a) * C + m]
+    size_t at(int a, int b, int m) const { return ((size_t)b * na + a) * C + m; }
+    // Largest projected density, folded as the reference's nested std::max (utils.h:199).
+    double max_density() const
+    {
+        double mx = 0.0;
+        for (size_t k = 0; k < n.size(); k += C) {
+            double m = n[k + C - 1];
+            for (int c = C - 2; c >= 0; --c) m = std::max(n[k + c], m);
+            mx = std::max(mx, m);
+        }
+        return mx;
+    }
+};
+
+// axis numbers 0 = x, 1 = y, 2 = z; a, b = the picture's horizontal and vertical axes.
+Plane project(const Level &L, int a, int b, int basis)
+{
+    const int P[3] = {L.nx + 2, L.ny + 2, L.nz + 2};
+    const int s = 3 - a - b;
+    Plane pl;
+    pl.na = P[a];
+    pl.nb = P[b];
+    pl.C = L.C;
+    const size_t cells = (size_t)pl.na * pl.nb * L.C;
+    pl.n.assign(cells, 0.0);
+    pl.r.assign(cells, 0.0);
+    pl.i.assign(cells, 0.0);
+    std::vector<double> tn((size_t)L.V * L.C), tr((size_t)L.V * L.C), ti((size_t)L.V * L.C);
+    double re[5], im[5];
+    for (int qb = 1; qb < P[b] - 1; ++qb)
+        for (int qa = 1; qa < P[a] - 1; ++qa) {
+            double *n = &pl.n[pl.at(qa, qb, 0)], *r = &pl.r[pl.at(qa, qb, 0)], *i = &pl.i[pl.at(qa, qb, 0)];
+            for (int qs = 1; qs < P[s] - 1; ++qs) {
+                int q[3];
+                q[a] = qa;
+                q[b] = qb;
+                q[s] = qs;
+                for (int v = 0; v < L.V; ++v) {
+                    spinor_at(L, q[0], q[1], q[2], v, re, im);
+                    if (L.C == 3) spin1_basis(basis, re, im);
+                    if (L.C == 5) spin2_basis(basis, re, im);
+                    for (int c = 0; c < L.C; ++c) {
+                        const double rr = re[c] * re[c], ii = im[c] * im[c];
+                        tr[v * L.C + c] = rr;
+                        ti[v * L.C + c] = ii;
+                        tn[v * L.C + c] = rr + ii;
+                    }
+                }
+                for (int rep = 0; rep < 2; ++rep)
+                    for (int v = 0; v < L.V; ++v)
+                        for (int c = 0; c < L.C; ++c) {
+                            n[c] += tn[v * L.C + c];
+                            r[c] += tr[v * L.C + c];
+                            i[c] += ti[v * L.C + c];
+                        }
+            }
+        }
+    return pl;
+}
+
+// Panels of one plane: component m's panel starts at x = m * pw; pixel (u, w) of the panel
+// shows block (u / 2, w / 2) of the plane at picture pixel (x0 + u', y0 + w') with u' = u or
+// pw - 1 - u (flip_u) and w' likewise.  ri: red = I * sum Re^2, green = I * sum Im^2; else a
+// grey level I * sum |psi_m|^2.
+void draw_plane(Picture &pic, const Plane &pl, int pw, int y0, double I, bool ri, bool flip_u, bool flip_w)
+{
+    const int nu = 2 * pl.na, nw = 2 * pl.nb;
+    for (int w = 0; w < nw; ++w)
+        for (int u = 0; u < nu; ++u)
+            for (int c = 0; c < pl.C; ++c) {
+                const size_t k = pl.at(u / 2, w / 2, c);
+                const int x = c * pw + (flip_u ? nu - 1 - u : u), y = y0 + (flip_w ? nw - 1 - w : w);
+                if (ri)
+                    pic.set(x, y, I * pl.r[k], I * pl.i[k], 0.0);
+                else {
+                    const double g = I * pl.n[k];
+                    pic.set(x, y, g, g, g);
+                }
+            }
+}
+
+// Grey separators: rows at y = rows[k], columns at x = k * pw (k = 1..panels-1).
+void separators(Picture &pic, int pw, int panels, const int *rows, int nrows)
+{
+    for (int k = 0; k < nrows; ++k)
+        for (int x = 0; x < pic.width(); ++x) pic.set(x, rows[k], 0.5, 0.5, 0.5);
+    for (int y = 0; y < pic.height(); ++y)
+        for (int k = 1; k < panels; ++k) pic.set(k * pw, y, 0.5, 0.5, 0.5);
+}
+
+// The reference's density pictures of a state, with W = 2 (nx + 2), H = 2 (ny + 2),
+// D = 2 (nz + 2) pixels along the padded grid:
+//   GPE_PICTURE_DENSITY, spin-1 (drawDensity, Spin1GpeBCC/utils.h:151-366): 3 W x 2 H; top
+//     row of panels (y < D) the column sums over y (XZ), bottom row (y >= H) the sums over z
+//     (XY), both mirrored in x and the XY panels in y; grey |psi_m|^2 / max of the plane.
+//   GPE_PICTURE_DENSITY, spin-2 (drawDensity, Spin2GpeBCC/utils.h:287-796): 5 W x 3 H; XZ,
+//     then YZ (sums over x, scaled by the XZ maximum as the reference does), then XY, unmirrored.
+//   GPE_PICTURE_RI, spin-1 (drawDensityRI, Spin1GpeBCC/utils.h:368-580): 3 W x 2 H, XZ and XY
+//     unmirrored, red = sum Re^2, green = sum Im^2, both times 2 / the plane's largest sum of
+//     |psi_m|^2.
+//   GPE_PICTURE_RI, spin-2 (drawIandR, Spin2GpeBCC/utils.h:131-285): 5 W x 2 H, XZ and XY,
+//     red = sum Re^2, green = sum Im^2 unscaled, components in the z basis.
+//   GPE_PICTURE_SLICE, scalar (GPUGrossPitaevskii_BCC.cu:700-723): W x H, the padded block
+//     layer z = nz / 2 + 1, unknown v = 0: red = 20 Re^2, green = 20 Im^2.
+int write_picture(const Level &L, const char *path, int style, int basis)
+{
+    if (basis < 0 || basis > 2) return GPE_EINVAL;
+    const int W = 2 * (L.nx + 2), H = 2 * (L.ny + 2);
+    if (style == GPE_PICTURE_SLICE) {
+        if (L.C != 1) return GPE_EINVAL;
+        Picture pic(W, H);
+        const int k = L.nz / 2 + 1;
+        for (int j = 0; j < H; ++j)
+            for (int i = 0; i < W; ++i) {
+                double re[1], im[1];
+                spinor_at(L, i / 2, j / 2, k, 0, re, im);
+                pic.set(i, j, 20.0 * (re[0] * re[0]), 20.0 * (im[0] * im[0]), 0.0);
+            }
+        return pic.save(path);
+    }
+    if ((style != GPE_PICTURE_DENSITY && style != GPE_PICTURE_RI) || L.C == 1) return GPE_EINVAL;
+    const bool ri = style == GPE_PICTURE_RI;
+    if (ri && L.C == 5 && basis != GPE_BASIS_Z) return GPE_EINVAL;
+    const Plane xz = project(L, 0, 2, basis);
+    const Plane xy = project(L, 0, 1, basis);
+    if (L.C == 3) {
+        Picture pic(3 * W, 2 * H);
+        const double num = ri ? 2.0 : 1.0;
+        draw_plane(pic, xz, W, 0, num / xz.max_density(), ri, !ri, false);
+        draw_plane(pic, xy, W, H, num / xy.max_density(), ri, !ri, !ri);
+        const int rows[1] = {H};
+        separators(pic, W, 3, rows, 1);
+        return pic.save(path);
+    }
+    if (ri) {
+        Picture pic(5 * W, 2 * H);
+        draw_plane(pic, xz, W, 0, 1.0, true, false, false);
+        draw_plane(pic, xy, W, H, 1.0, true, false, false);
+        const int rows[1] = {H};
+        separators(pic, W, 5, rows, 1);
+        return pic.save(path);
+    }
+    Picture pic(5 * W, 3 * H);
+    const double Ixz = 1.0 / xz.max_density();
+    draw_plane(pic, xz, W, 0, Ixz, false, false, false);
+    draw_plane(pic, project(L, 1, 2, basis), W, H, Ixz, false, false, false);
+    draw_plane(pic, xy, W, 2 * H, 1.0 / xy.max_density(), false, false, false);
+    const int rows[2] = {H, 2 * H};
+    separators(pic, W, 5, rows, 2);
+    return pic.save(path);
+}
+
 std::string join(const char *dir, const char *prefix, const std::string &name)
 {
     std::string p = dir && *dir ? std::string(dir) + "/" : std::string();
@@ -591,6 +846,22 @@ int gpe_write_vtk_volume(gpe_ctx *ctx, int level, const char *path, int basis)
     Level L;
     int rc = fetch(ctx, level, L);
     return rc ? rc : write_volume(L, path, basis);
+}
+
+int gpe_write_bmp(gpe_ctx *ctx, int level, const char *path, int style, int basis)
+{
+    if (!ctx || !path || (level != 0 && level != 1)) return GPE_EINVAL;
+    Level L;
+    int rc = fetch(ctx, level, L);
+    return rc ? rc : write_picture(L, path, style, basis);
+}
+
+int gpe_host_write_bmp(const gpe_config *cfg, const double *psi, const char *path, int style, int basis)
+{
+    Level L;
+    if (!cfg || !psi || !path || shape_of(*cfg, L)) return GPE_EINVAL;
+    L.psi = psi;
+    return write_picture(L, path, style, basis);
 }
 
 }  // extern "C"

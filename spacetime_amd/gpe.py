@@ -27,6 +27,7 @@ OBJ_OVERLAP, OBJ_ENERGY, OBJ_MAGNETIZATION_Z = 0, 1, 2
 FIELD_DENSITY, FIELD_SPIN, FIELD_SPIN_NORM, FIELD_UV_THETA = 0, 1, 2, 3
 PHASES = {"polar": 0, "ferro": 1, "mixed": 2, "un": 3, "bn_hori": 4, "bn_vert": 5, "cyclic": 6}
 BASIS_Z, BASIS_Y, BASIS_X = 0, 1, 2
+PICTURE_DENSITY, PICTURE_RI, PICTURE_SLICE = 0, 1, 2
 ERRORS = {-22: "EINVAL", -12: "ENOMEM", -5: "EDEVICE", -71: "ESTATE", -34: "ERANGE"}
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -153,6 +154,8 @@ def load_library(path=LIB_PATH):
     lib.gpe_host_write_vtk_spinor.argtypes = [cp, vp, ctypes.c_char_p, ctypes.c_int]
     lib.gpe_host_write_vtk_preimage.argtypes = [cp, vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_double]
     lib.gpe_host_write_vtk_volume.argtypes = [cp, vp, ctypes.c_char_p, ctypes.c_int]
+    lib.gpe_write_bmp.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
+    lib.gpe_host_write_bmp.argtypes = [cp, vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
     _LIB = lib
     return lib
 
@@ -408,6 +411,9 @@ class Stepper:
 
     def write_vtk_volume(self, path, level=0, basis=BASIS_Z):
         self._check(self.lib.gpe_write_vtk_volume(self._h, level, os.fsencode(path), basis), "gpe_write_vtk_volume")
+
+    def write_bmp(self, path, level=0, style=PICTURE_DENSITY, basis=BASIS_Z):
+        self._check(self.lib.gpe_write_bmp(self._h, level, os.fsencode(path), style, basis), "gpe_write_bmp")
 
     def start(self, mode=START_EULER, mu=0.0):
         self._check(self.lib.gpe_start(self._h, mode, mu), "gpe_start")

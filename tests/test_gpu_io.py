@@ -4,7 +4,7 @@
   context, M more steps == N + M uninterrupted steps, bit for bit (FP64 and FP32).
 * Device spinor projections (Spin1GpeBCC.cu:463-551, Spin2GpeBCC.cu:417-558) and noise against
   host restatements; knot ramps against the equivalent ramp segments, bit for bit.
-* VTK and .dat files written from a context equal the host writers' files for the same state.
+* VTK, .dat and BMP files written from a context equal the host writers' files for the same state.
 * The adjoint of an FP32 context with FP64 psi and mu equals an FP64 context's adjoint.
 """
 import os
@@ -173,6 +173,27 @@ def test_vtk_from_a_context_matches_the_host_writer(tmp_path, ncomp, kinds):
         dev = (tmp_path / ("dev_%s.vtk" % kind)).read_bytes()
         assert dev == (tmp_path / ("host_%s.vtk" % kind)).read_bytes(), kind
         assert len(dev) > 200
+
+
+@pytest.mark.parametrize("ncomp,styles", [(1, [2]), (3, [0, 1]), (5, [0, 1])])
+def test_pictures_from_a_context_match_the_host_writer(tmp_path, ncomp, styles):
+    """gpe_write_bmp of a level after steps equals gpe_host_write_bmp of the state read back."""
+    from spacetime_amd import pictures
+    cfg = _cfg(gpe.FP64, ncomp=ncomp, dims=(5, 4, 3))
+    with gpe.Stepper(cfg) as st:
+        st.set_state(normalized(random_state(cfg.shape, 21), cfg))
+        st.start(gpe.START_EULER)
+        st.step(4)
+        psi = st.get_state(1)
+        for style in styles:
+            for basis in ([gpe.BASIS_Z] if ncomp == 1 or style == 1 and ncomp == 5 else [gpe.BASIS_Z, gpe.BASIS_Y]):
+                name = "%d_%d" % (style, basis)
+                pictures.save(st, tmp_path / ("dev_%s.bmp" % name), level=1, style=style, basis=basis)
+                pictures.write(tmp_path / ("host_%s.bmp" % name), cfg, psi, style, basis)
+                dev = (tmp_path / ("dev_%s.bmp" % name)).read_bytes()
+                assert dev == (tmp_path / ("host_%s.bmp" % name)).read_bytes(), name
+                _, rgb = pictures.read(tmp_path / ("dev_%s.bmp" % name))
+                assert rgb.max() > 0
 
 
 def test_fp64_trajectory_adjoint_of_an_fp32_context():
