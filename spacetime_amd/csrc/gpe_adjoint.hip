@@ -478,7 +478,7 @@ __device__ __forceinline__ void adjoint_pair(const AdjArgs &a, int b, int nb, do
 #pragma unroll
         for (int f = 0; f < DEG; ++f) {
             wf[f] = wt[f];
-            nbase[f] = own + fd[f] + (roff + scf[f]);
+            nbase[f] = own + (fd[f] + roff + scf[f]);  // one 32-bit lane offset
         }
     } else {
 #pragma unroll

@@ -72,11 +72,13 @@ struct WaveCoord {
     bool valid;
 };
 
+// n / d for n < 2^31 (wave and lane indices; the geometry keeps them there): with
+// m = floor(2^32 (2^l - d) / d) + 1, t = mulhi(n, m) <= n, so t + n does not wrap and
+// (t + n) >> l is the quotient; d = 1 has m = 1, l = 0 and gives n.  Three scalar
+// instructions and no branch.
 __device__ __forceinline__ unsigned fast_div(unsigned n, const FastDiv &f)
 {
-    if (f.d == 1) return n;
-    const unsigned t = __umulhi(n, f.m);
-    return (t + ((n - t) >> 1)) >> (f.l - 1);
+    return (__umulhi(n, f.m) + n) >> f.l;
 }
 
 // Wave coordinates of workgroup b of nb (b, nb: the workgroup's place in its own run) for a

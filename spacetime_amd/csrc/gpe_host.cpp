@@ -245,7 +245,7 @@ FastDiv make_fast_div(unsigned d)
 }
 
 // Segments, tiles, waves and workgroups of a traversal from nx, ny, nz, V, ty and xpl.
-// Wave indices must fit in 32 bits (the decode's arithmetic).
+// Wave indices must stay below 2^31 (the decode's divisions).
 // Lanes of a traversal's waves that carry values: rows * pairs pair slots in whole waves.
 static double lane_use(int rows, int pairs)
 {
@@ -285,7 +285,7 @@ bool finish_geometry(Geometry &g)
     g.ntile = (g.ny + g.ty - 1) / g.ty;
     g.waves = (long long)g.ntile * g.tyr * g.nz * g.V * g.nseg;
     const long long blocks = (g.waves + 3) / 4;
-    if (blocks >= (1ll << 30)) return false;
+    if (blocks >= (1ll << 29)) return false;  // wave indices below 2^31 (the decode's divisions)
     g.blocks = (int)blocks;
     g.f_nseg = make_fast_div((unsigned)g.nseg);
     g.f_V = make_fast_div((unsigned)g.V);
@@ -1553,7 +1553,7 @@ int gpe_ensemble_timing(const gpe_ensemble *e, double *ms_per_sweep, double *unk
 
 // ------------------------------------------------------------------------ autotuning ---
 
-// Apply a traversal to both geometries; on failure (a workgroup count past the 2^30 limit of
+// Apply a traversal to both geometries; on failure (a workgroup count past the 2^29 limit of
 // the decode) both are left as they were and false is returned.
 static bool set_traversal(gpe_ctx *c, int ty, int remap, int xpl)
 {

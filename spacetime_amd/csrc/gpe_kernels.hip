@@ -359,7 +359,7 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
 #pragma unroll
             for (int f = 0; f < DEG; ++f) {
                 wf[f] = wt[f];
-                nbase[f] = own + fd[f] + (roff + scf[f]);
+                nbase[f] = own + (fd[f] + roff + scf[f]);  // one 32-bit lane offset
             }
         } else {
 #pragma unroll

@@ -32,13 +32,36 @@ def _plan(cfg, ty, remap, xpl):
 
 
 def _fdiv(n, div):
-    """The device's n / d as multiply-high and shifts (gpe_device.h fast_div), on uint32 n."""
+    """The device's n / d as multiply-high, add and shift (gpe_device.h fast_div) for n < 2^31,
+    in 32-bit unsigned arithmetic."""
     d, m, l = (int(x) for x in div)
     n = n.astype(np.uint64)
-    if d == 1:
-        return n
+    assert int(n.max(initial=0)) < 2 ** 31
     t = (n * np.uint64(m)) >> np.uint64(32)
-    return (t + ((n - t) >> np.uint64(1))) >> np.uint64(l - 1)
+    s = t + n
+    assert int(s.max(initial=0)) < 2 ** 32  # no 32-bit wrap
+    return s >> np.uint64(l)
+
+
+def _magic(d):
+    """The host's constants for d (gpe_host.cpp make_fast_div), restated."""
+    l = 0
+    while (1 << l) < d:
+        l += 1
+    m = 1 if d <= 1 else ((2 ** 32) * ((1 << l) - d)) // d + 1
+    return d, m, l
+
+
+@pytest.mark.parametrize("d", [1, 2, 3, 5, 7, 12, 41, 46, 64, 80, 97, 136, 1000, 4099, 65536, 1000003, 2 ** 30 + 3,
+                               2 ** 31 - 1])
+def test_fast_division_is_exact_below_2_31(d):
+    rng = np.random.default_rng(d)
+    k = rng.integers(0, (2 ** 31 - 1) // d + 1, 2000)  # multiples of d and their neighbours
+    n = np.concatenate([rng.integers(0, 2 ** 31, 20000), np.arange(0, min(4 * d + 5, 5000)),
+                        k * d, k * d + d - 1, np.maximum(k * d, 1) - 1,
+                        [2 ** 31 - 1, 2 ** 31 - 2, (2 ** 31 - 1) // d * d, (2 ** 31 - 1) // d * d - 1]])
+    n = n[(n >= 0) & (n < 2 ** 31)].astype(np.uint64)
+    np.testing.assert_array_equal(_fdiv(n, _magic(d)).astype(np.int64), n.astype(np.int64) // d)
 
 
 def _logical_block(b, nb):
@@ -58,7 +81,7 @@ def _decode_all(p):
     b = np.arange(nb, dtype=np.int64)
     lb = _logical_block(b, nb) if p.xcd_remap else b
     w = (lb[:, None] * 4 + np.arange(4)[None, :]).reshape(-1).astype(np.uint64)
-    assert int(w.max()) < 2 ** 32
+    assert int(w.max()) < 2 ** 31
     q0 = _fdiv(w, p.div_nseg)
     xseg = (w - q0 * np.uint64(p.nseg)).astype(np.int64)
     q1 = _fdiv(q0, p.div_V)
