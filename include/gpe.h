@@ -247,8 +247,8 @@ int gpe_traversal(const gpe_ctx *ctx, int *tile_rows, int *xcd_remap, int *x_per
  * without a device: workgroups of four waves, each wave a 64-lane segment of one (z, y, v)
  * x-line (rows = 1) or of a group of `rows` y rows (x pairs spread over the rows), in the
  * order y-tile, z, row (group) in tile, unknown v, segment; and the constants of the wave
- * decode's divisions by invariants (d, m, l: n / d = (t + ((n - t) >> 1)) >> (l - 1), t the
- * high 32 bits of n m; d = 1 passes n through).  No counterpart in the reference, whose
+ * decode's divisions by invariants (d, m, l: n / d = (t + n) >> l for n < 2^31, t the high
+ * 32 bits of n m).  No counterpart in the reference, whose
  * launch shape is fixed (Spin1GpeBCC.cu:1376-1380). */
 typedef struct {
     int nx, ny, nz, V, C;
