@@ -620,10 +620,10 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs a)
     else
         adjoint_pair<C, DEG, X == 3>(a, b, gridDim.x, part);
     __shared__ double red[4][ADJ_COUNT];
+    wave_sum_n<ADJ_COUNT>(part);
+    if ((threadIdx.x & 63) == 0) {
 #pragma unroll
-    for (int i = 0; i < ADJ_COUNT; ++i) {
-        const double r = wave_sum(part[i]);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][i] = r;
+        for (int i = 0; i < ADJ_COUNT; ++i) red[threadIdx.x >> 6][i] = part[i];
     }
     __syncthreads();
     if (threadIdx.x < ADJ_COUNT) {

@@ -324,6 +324,21 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// N wave sums at once: each butterfly step exchanges all N values before any is added, so a
+// wave waits on N lane exchanges once per step instead of once per step and value.
+template <int N>
+__device__ __forceinline__ void wave_sum_n(double *v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        double o[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) o[i] = __shfl_xor(v[i], off, 64);
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] += o[i];
+    }
+}
+
 __device__ __forceinline__ double wave_max(double v)
 {
 #pragma unroll
