@@ -269,7 +269,8 @@ def run_lattices(args, log):
     """FP32 sweep rate of every lattice x component count at ~25M unknowns (the spin-1 BCC 128^3
     size): cube 293^3, BCC / FCC 128^3, A15 82^3, C15 57^3, Z 68^3 blocks; autotuned traversal.  Each case
     also runs the plain FP64 form from the same start (unless --no-fp64) and reports the relative
-    L2 distance of the final states and both forms' norm and energy drift."""
+    L2 distance of the final states and both forms' norm and energy drift.  --cases picks cases and
+    may give a case another box (block scale box / nx)."""
     from spacetime_amd import gpe, lattice
     import dataclasses
     box = 16.0
@@ -278,82 +279,85 @@ def run_lattices(args, log):
     target = 25165824
     cases = []
     tot_nodes_steps, tot_s, tot_bytes, tot_kernel_s = 0.0, 0.0, 0.0, 0.0
-    only = None
-    if args.cases:  # "lattice:C,lattice:C": a subset of the cases (profiling runs)
-        only = {(c.split(":")[0], int(c.split(":")[1])) for c in args.cases.split(",")}
-    for name in ("cube", "bcc", "fcc", "a15", "c15", "z"):
+    if args.cases:  # "lattice:C[:NXxNYxNZ],...": chosen cases, optionally on another box (profiling runs)
+        plan = []
+        for c in args.cases.split(","):
+            parts = c.split(":")
+            plan.append((parts[0], int(parts[1]), tuple(int(v) for v in parts[2].split("x")) if len(parts) > 2 else None))
+    else:
+        plan = [(name, ncomp, None) for name in ("cube", "bcc", "fcc", "a15", "c15", "z") for ncomp in (1, 3, 5)]
+    for name, ncomp, dims in plan:
         lat = lattice.get(name)
         n = int(round((target / lat.V) ** (1.0 / 3.0)))
-        for ncomp in (1, 3, 5):
-            if only is not None and (name, ncomp) not in only:
-                continue
-            kw = dict(c0=gmu)
-            if ncomp == 3:
-                kw.update(c1=500.0, q=0.1)
-            elif ncomp == 5:
-                kw.update(c1=200.0, c2=50.0)
-            cell = box / n
-            probe = gpe.Config.centred_box(name, (n, n, n), cell, ncomp=ncomp, precision=gpe.FP32, dt=1e-4, **kw)
-            # the 3-level leapfrog needs dt * ||H|| < 1
-            dt = min(1e-4, 0.5 / (lat.kinetic_bound(probe.block_scale) + mu))
-            cfg = gpe.Config.centred_box(name, (n, n, n), cell, ncomp=ncomp, precision=gpe.FP32, dt=dt, **kw)
-            weights = np.zeros(ncomp, dtype=np.complex128)
-            weights[ncomp // 2] = 1.0
-            field = dict(bq=4e-3, bb=(0.0, 0.0, 0.05)) if ncomp > 1 else None
-            cfg64 = dataclasses.replace(cfg, precision=gpe.FP64)
-            with gpe.Stepper(cfg64) as st:  # the common start: FP64 Thomas-Fermi state with noise
-                st.fill_thomas_fermi(gmu, mu, weights, noise=0.005, seed=17)
-                st.scale(1.0 / np.sqrt(st.observables(0).norm))
-                psi0 = st.get_state(0)
+        dims = dims or (n, n, n)
+        kw = dict(c0=gmu)
+        if ncomp == 3:
+            kw.update(c1=500.0, q=0.1)
+        elif ncomp == 5:
+            kw.update(c1=200.0, c2=50.0)
+        cell = box / dims[0]
+        probe = gpe.Config.centred_box(name, dims, cell, ncomp=ncomp, precision=gpe.FP32, dt=1e-4, **kw)
+        # the 3-level leapfrog needs dt * ||H|| < 1
+        dt = min(1e-4, 0.5 / (lat.kinetic_bound(probe.block_scale) + mu))
+        cfg = gpe.Config.centred_box(name, dims, cell, ncomp=ncomp, precision=gpe.FP32, dt=dt, **kw)
+        weights = np.zeros(ncomp, dtype=np.complex128)
+        weights[ncomp // 2] = 1.0
+        field = dict(bq=4e-3, bb=(0.0, 0.0, 0.05)) if ncomp > 1 else None
+        cfg64 = dataclasses.replace(cfg, precision=gpe.FP64)
+        with gpe.Stepper(cfg64) as st:  # the common start: FP64 Thomas-Fermi state with noise
+            st.fill_thomas_fermi(gmu, mu, weights, noise=0.005, seed=17)
+            st.scale(1.0 / np.sqrt(st.observables(0).norm))
+            psi0 = st.get_state(0)
 
-            def run(c, timed):
-                with gpe.Stepper(c) as st:
-                    if field:
-                        st.set_field(**field)
-                    st.set_state(psi0)
-                    o0 = st.observables(0)
-                    st.start(gpe.START_EULER)
-                    if timed and args.traversal:
-                        st.set_traversal(*args.traversal)
-                        tune = st.traversal()
-                    else:
-                        tune = st.autotune(args.tune_sweeps) if (timed and args.tune_sweeps) else st.traversal()
-                    st.step(args.warmup)
-                    st.sync()
-                    t0 = time.perf_counter()
-                    st.step(args.steps)
-                    st.sync()
-                    wall = time.perf_counter() - t0
-                    ms, nb = st.last_step_timing()
-                    o1 = st.observables(0)
-                    psi = st.get_state(0)
-                drift = dict(norm_drift=abs(o1.norm - o0.norm) / o0.norm,
-                             energy_drift=abs(o1.energy - o0.energy) / abs(o0.energy))
-                return tune, wall, ms, nb, o1, psi, drift
+        def run(c, timed):
+            with gpe.Stepper(c) as st:
+                if field:
+                    st.set_field(**field)
+                st.set_state(psi0)
+                o0 = st.observables(0)
+                st.start(gpe.START_EULER)
+                if timed and args.traversal:
+                    st.set_traversal(*args.traversal)
+                    tune = st.traversal()
+                else:
+                    tune = st.autotune(args.tune_sweeps) if (timed and args.tune_sweeps) else st.traversal()
+                st.step(args.warmup)
+                st.sync()
+                t0 = time.perf_counter()
+                st.step(args.steps)
+                st.sync()
+                wall = time.perf_counter() - t0
+                ms, nb = st.last_step_timing()
+                o1 = st.observables(0)
+                psi = st.get_state(0)
+            drift = dict(norm_drift=abs(o1.norm - o0.norm) / o0.norm,
+                         energy_drift=abs(o1.energy - o0.energy) / abs(o0.energy))
+            return tune, wall, ms, nb, o1, psi, drift
 
-            tune, wall, ms, nb, o, psi32, drift32 = run(cfg, True)
-            acc = dict(norm_drift_fp32=drift32["norm_drift"], energy_drift_fp32=drift32["energy_drift"])
-            if not args.no_fp64:  # the plain FP64 form from the same start, same steps
-                _, _, _, _, _, psi64, drift64 = run(cfg64, False)
-                acc.update(rel_l2_fp32_vs_fp64=float(np.linalg.norm((psi32 - psi64).ravel()) /
-                                                     np.linalg.norm(psi64.ravel())),
-                           norm_drift_fp64=drift64["norm_drift"], energy_drift_fp64=drift64["energy_drift"])
-                del psi64
-            del psi32, psi0
-            nodes = float(n ** 3 * lat.V)
-            gbs = nb / (ms * 1e-3) / 1e9
-            cases.append(dict(lattice=name, components=ncomp, blocks=n, unknowns=nodes, dt=dt,
-                              traversal=[tune["tile_rows"], tune["xcd_remap"], tune["x_per_lane"]],
-                              kernel_ms_per_sweep=ms, node_updates_per_s=nodes * args.steps / wall,
-                              achieved_gbs=gbs, frac=gbs / PEAK_HBM_GBS, norm=o.norm, finite=bool(o.finite),
-                              accuracy=acc))
-            log("%-4s C=%d %3d^3 blocks: %.4f ms/sweep, %.3e node-updates/s, %6.1f GB/s (%.1f%%), traversal %s%s" % (
-                name, ncomp, n, ms, nodes / (ms * 1e-3), gbs, 100 * gbs / PEAK_HBM_GBS, cases[-1]["traversal"],
-                (", FP32 vs FP64 rel L2 %.2e" % acc["rel_l2_fp32_vs_fp64"]) if "rel_l2_fp32_vs_fp64" in acc else ""))
-            tot_nodes_steps += nodes * args.steps
-            tot_s += wall
-            tot_bytes += nb * args.steps
-            tot_kernel_s += ms * 1e-3 * args.steps
+        tune, wall, ms, nb, o, psi32, drift32 = run(cfg, True)
+        acc = dict(norm_drift_fp32=drift32["norm_drift"], energy_drift_fp32=drift32["energy_drift"])
+        if not args.no_fp64:  # the plain FP64 form from the same start, same steps
+            _, _, _, _, _, psi64, drift64 = run(cfg64, False)
+            acc.update(rel_l2_fp32_vs_fp64=float(np.linalg.norm((psi32 - psi64).ravel()) /
+                                                 np.linalg.norm(psi64.ravel())),
+                       norm_drift_fp64=drift64["norm_drift"], energy_drift_fp64=drift64["energy_drift"])
+            del psi64
+        del psi32, psi0
+        nodes = float(np.prod(dims) * lat.V)
+        gbs = nb / (ms * 1e-3) / 1e9
+        cases.append(dict(lattice=name, components=ncomp, blocks=list(dims), unknowns=nodes, dt=dt,
+                          traversal=[tune["tile_rows"], tune["xcd_remap"], tune["x_per_lane"]],
+                          kernel_ms_per_sweep=ms, node_updates_per_s=nodes * args.steps / wall,
+                          achieved_gbs=gbs, frac=gbs / PEAK_HBM_GBS, norm=o.norm, finite=bool(o.finite),
+                          accuracy=acc))
+        log("%-4s C=%d %s blocks: %.4f ms/sweep, %.3e node-updates/s, %6.1f GB/s (%.1f%%), traversal %s%s" % (
+            name, ncomp, "x".join(str(d) for d in dims), ms, nodes / (ms * 1e-3), gbs, 100 * gbs / PEAK_HBM_GBS,
+            cases[-1]["traversal"],
+            (", FP32 vs FP64 rel L2 %.2e" % acc["rel_l2_fp32_vs_fp64"]) if "rel_l2_fp32_vs_fp64" in acc else ""))
+        tot_nodes_steps += nodes * args.steps
+        tot_s += wall
+        tot_bytes += nb * args.steps
+        tot_kernel_s += ms * 1e-3 * args.steps
     achieved = tot_bytes / tot_kernel_s / 1e9
     return {
         "metric": "FP32 node-updates/s of every lattice x component count at ~25M unknowns",
@@ -680,7 +684,8 @@ def main():
     ap.add_argument("--traversal", type=lambda v: tuple(int(x) for x in v.split(",")), default=None,
                     help="fixed FP32 sweep traversal 'rows,remap,x_per_lane' instead of the autotuner")
     ap.add_argument("--cases", default=None,
-                    help="lattices workload: only these cases, 'lattice:C,...' (e.g. 'a15:5,bcc:1')")
+                    help="lattices workload: only these cases, 'lattice:C[:NXxNYxNZ],...' (e.g. 'a15:5,bcc:1' or "
+                         "'a15:5:64x82x105' for another box)")
     ap.add_argument("--itp-steps", type=int, default=1000)
     ap.add_argument("--itp-dt", type=float, default=2e-4)
     ap.add_argument("--no-cpu", action="store_true")
