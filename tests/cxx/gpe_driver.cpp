@@ -2,7 +2,7 @@
 // integrateInTime (Spin1GpeBCC.cu:1105-1743): set-up, a device-side start (Thomas-Fermi fill,
 // imaginary time, polar projection, spinor noise, renormalisation), a knot ramp, forward-Euler
 // start and leapfrog steps, a checkpoint pair written mid-run and resumed in a second context,
-// observables.  Prints one JSON line: the observables of both runs, whether the resumed run
+// observables and a density picture.  Prints one JSON line: the observables of both runs, whether the resumed run
 // equals the uninterrupted one bit for bit, and the Hamiltonian bound.
 //
 // usage: gpe_driver <directory for the checkpoint files>
@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #define CHECK(ctx, call)                                                                         \
@@ -89,6 +90,9 @@ int main(int argc, char **argv)
     CHECK(a, gpe_max_hamiltonian(a, 0, &hmax));
     std::vector<double> fa(2 * n);
     CHECK(a, gpe_get_state(a, 0, fa.data()));
+    // the density picture the reference draws at each save (drawDensity, Spin1GpeBCC.cu:1557)
+    const std::string picture = std::string(dir) + "/drv_" + tag + "ms.bmp";
+    CHECK(a, gpe_write_bmp(a, 0, picture.c_str(), GPE_PICTURE_DENSITY, GPE_BASIS_Z));
 
     gpe_ctx *b = nullptr;
     if (gpe_create(&cfg, &b) != GPE_OK) {

@@ -1,7 +1,7 @@
 """The C ABI driven from C++ alone: tests/cxx/gpe_driver (built with the library) runs a spin-1
-set-up, a device-side start, a knot ramp, leapfrog steps and a checkpoint pair resumed in a
-second context; its resumed run equals the uninterrupted one bit for bit, and its observables
-equal those of the same sequence through the Python binding."""
+set-up, a device-side start, a knot ramp, leapfrog steps, a checkpoint pair resumed in a second
+context and a density picture; its resumed run equals the uninterrupted one bit for bit, and its
+observables equal those of the same sequence through the Python binding."""
 import json
 import os
 import subprocess
@@ -44,6 +44,9 @@ def test_cxx_driver_resumes_bitwise_and_matches_python(tmp_path):
     out = json.loads(r.stdout.strip().splitlines()[-1])
     assert out["bitwise_resume"] is True and out["levels"] == 2
     assert (tmp_path / ("drv_%s.dat" % out["tag"])).exists() and (tmp_path / ("drv_even_%s.dat" % out["tag"])).exists()
+    from spacetime_amd import pictures
+    header, rgb = pictures.read(tmp_path / ("drv_%sms.bmp" % out["tag"]))
+    assert (header["width"], header["height"]) == (3 * 2 * 14, 2 * 2 * 12) and rgb.max() == 255
     assert out["norm_resumed"] == out["norm"] and out["energy_resumed"] == out["energy"]
     assert np.isfinite(out["max_hamiltonian"]) and out["max_hamiltonian"] > 0
     o = _python_run()
