@@ -49,8 +49,8 @@ static hipError_t scratch_address(int device, void **out)
 
 // ABI elements [first, first + count) of a level (ABI order c, z, y, x, v) into out[0, count).
 template <typename R>
-__global__ __launch_bounds__(256) void k_unpack_range(Geometry g, int ncomp, const vec2_t<R> *dev, double2 *out,
-                                                      long long first, long long count)
+__global__ __launch_bounds__(256) void k_unpack_range(Geometry g, const vec2_t<R> *dev, double2 *out, long long first,
+                                                      long long count)
 {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= count) return;
@@ -88,10 +88,10 @@ extern "C" int gpe_get_state_noalloc(gpe_ctx *ctx, int level, double *psi)
         const long long n = total - first < chunk ? total - first : chunk;  // n <= chunk: within the scratch
         const unsigned grid = (unsigned)((n + 255) / 256);
         if (cv.fp64)
-            hipLaunchKernelGGL(k_unpack_range<double>, dim3(grid), dim3(256), 0, cv.stream, g, cv.ncomp,
+            hipLaunchKernelGGL(k_unpack_range<double>, dim3(grid), dim3(256), 0, cv.stream, g,
                                (const double2 *)cv.level[level], stage, first, n);
         else
-            hipLaunchKernelGGL(k_unpack_range<float>, dim3(grid), dim3(256), 0, cv.stream, g, cv.ncomp,
+            hipLaunchKernelGGL(k_unpack_range<float>, dim3(grid), dim3(256), 0, cv.stream, g,
                                (const float2 *)cv.level[level], stage, first, n);
         e = hipGetLastError();
         if (e == hipSuccess)
