@@ -202,6 +202,16 @@ def test_workgroup_orders_are_permutations(nb):
         assert np.all(np.diff(lb[sel]) < 0) or sel.sum() <= 1
 
 
+def test_plan_keeps_wave_indices_below_2_31():
+    """The decode's divisions are exact for wave indices below 2^31: a launch of 2^29 workgroups or
+    more (four waves each) is refused; one just below is planned."""
+    ok = gpe.Config(lattice="cube", ncomp=1, precision=gpe.FP64, dims=(64, 32768, 65535))
+    rc, p = _plan(ok, 1, 1, 1)
+    assert rc == 0 and p.waves < 2 ** 31 and p.blocks < 2 ** 29
+    big = gpe.Config(lattice="cube", ncomp=1, precision=gpe.FP64, dims=(64, 32768, 65537))
+    assert _plan(big, 1, 1, 1)[0] == -22
+
+
 def test_plan_rejects_bad_arguments():
     cfg = gpe.Config(lattice="bcc", ncomp=3, precision=gpe.FP64, dims=(8, 8, 8))
     assert _plan(cfg, 8, 1, 2)[0] == -22  # FP64 has one x per lane
