@@ -115,7 +115,29 @@ typedef struct {
                               reference's local spin z part is not divided by n, :258-261) */
     double moment[3];      /* sum dv |psi|^2 r: the centre of mass of a unit-norm state
                               (GPUGrossPitaevskii_BCC.cu:349-413, which copies back only x) */
+    double ref_spin;       /* the reference's "Spin" (integrateSpinAndDensity, Spin1GpeBCC.cu:
+                              992-1020): sum dv n |s| with s = (x, y of the spinor divided by
+                              sqrt n, z = |psi_1|^2 - |psi_-1|^2 undivided) (:235-271); spin-2
+                              takes the same expressions on m = 1, 0, -1 (Spin2GpeBCC.cu:214-262);
+                              0 for one component; unknowns with n = 0 add 0 (the reference NaN) */
+    double ref_magnetization_z; /* the reference's mag_z: sum dv n (|psi_1|^2 - |psi_-1|^2); its
+                              mag_x, mag_y are magnetization[0], [1] */
 } gpe_observables_t;
+
+/* The reference's time-series file of a run (Spin1GpeBCC.cu:1497-1504, 1659-1712): each append
+ * adds one column -- the level's time times the ramp's time scale (the reference's ms clock; 1
+ * without a ramp), the field's Bq and Bb x + y + z at that time, ref_spin, magnetization[0],
+ * magnetization[1], ref_magnetization_z and norm -- and each save writes
+ *   times = [times, t1, t2, ...];  Bq = [Bq, ...];  Bz = [Bz, ...];  Spin = [Spin, ...];
+ *   mag_x = [mag_x, ...];  mag_y = [...];  mag_z = [...];  norm = [norm, ...];
+ * one per line, values as "%.18f" (the reference's toString), then starts a new series, so
+ * running the saved files in order concatenates them as the reference's do.  append returns
+ * GPE_ERANGE (after appending) for a non-finite state. */
+typedef struct gpe_series gpe_series;
+int gpe_series_create(gpe_series **out);
+int gpe_series_destroy(gpe_series *series);
+int gpe_series_append(gpe_series *series, gpe_ctx *ctx, int level);
+int gpe_series_save(gpe_series *series, const char *path);
 
 /* Create a context: allocates both time levels on the current HIP device and uploads
  * the lattice tables.  Replaces integrateInTime's set-up (Spin1GpeBCC.cu:1105-1172). */

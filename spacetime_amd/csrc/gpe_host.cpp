@@ -774,7 +774,67 @@ int gpe_observables(gpe_ctx *c, int level, gpe_observables_t *out)
     out->moment[0] = h[OBS_RX];
     out->moment[1] = h[OBS_RY];
     out->moment[2] = h[OBS_RZ];
+    out->ref_spin = h[OBS_REF_SPIN];
+    out->ref_magnetization_z = h[OBS_REF_MZ];
     return out->finite ? GPE_OK : fail(c, GPE_ERANGE, "non-finite value in the state");
+}
+
+// ------------------------------------------------------------------ observable series --
+
+// The reference's time-series file (Spin1GpeBCC.cu:1497-1504, 1659-1712): eight MATLAB
+// vectors that each saved file extends ("times = [times, ...];"), one column per append.
+struct gpe_series {
+    std::vector<double> col[8];  // times, Bq, Bz, Spin, mag_x, mag_y, mag_z, norm
+};
+
+int gpe_series_create(gpe_series **out)
+{
+    if (!out) return GPE_EINVAL;
+    *out = new (std::nothrow) gpe_series();
+    return *out ? GPE_OK : GPE_ENOMEM;
+}
+
+int gpe_series_destroy(gpe_series *s)
+{
+    delete s;
+    return GPE_OK;
+}
+
+int gpe_series_append(gpe_series *s, gpe_ctx *c, int level)
+{
+    if (!s || !c) return GPE_EINVAL;
+    gpe_observables_t o;
+    const int rc = gpe_observables(c, level, &o);
+    if (rc && rc != GPE_ERANGE) return rc;
+    const FieldRec f = field_at(c, o.time);
+    const double v[8] = {o.time * (c->ramp.active ? c->ramp.time_scale : 1.0),
+                         f.bq,
+                         f.bb[0] + f.bb[1] + f.bb[2],
+                         o.ref_spin,
+                         o.magnetization[0],
+                         o.magnetization[1],
+                         o.ref_magnetization_z,
+                         o.norm};
+    for (int k = 0; k < 8; ++k) s->col[k].push_back(v[k]);
+    return rc;
+}
+
+int gpe_series_save(gpe_series *s, const char *path)
+{
+    if (!s || !path) return GPE_EINVAL;
+    static const char *const names[8] = {"times", "Bq", "Bz", "Spin", "mag_x", "mag_y", "mag_z", "norm"};
+    FILE *f = std::fopen(path, "w");
+    if (!f) return GPE_EINVAL;
+    bool ok = true;
+    for (int k = 0; k < 8 && ok; ++k) {
+        ok = std::fprintf(f, "%s = [%s", names[k], names[k]) > 0;
+        for (double x : s->col[k]) ok = ok && std::fprintf(f, ", %.18f", x) > 0;
+        ok = ok && std::fprintf(f, "];\n") > 0;
+    }
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) return GPE_EDEVICE;
+    for (auto &v : s->col) v.clear();  // the next file starts a new series, as the reference's does
+    return GPE_OK;
 }
 
 // k_aux over a level with an optional device reference state; h[0] = maximum, h[1] = sum.

@@ -131,7 +131,7 @@ struct SweepArgs {
 // Observables partial sums per workgroup.
 enum {
     OBS_NORM = 0, OBS_ENERGY, OBS_EXPECT_H, OBS_MX, OBS_MY, OBS_MZ, OBS_MAXDENS, OBS_NONFINITE,
-    OBS_SPIN, OBS_RX, OBS_RY, OBS_RZ, OBS_COUNT
+    OBS_SPIN, OBS_RX, OBS_RY, OBS_RZ, OBS_REF_SPIN, OBS_REF_MZ, OBS_COUNT
 };
 enum { FIELD_DENSITY = 0, FIELD_SPIN = 1, FIELD_SPIN_NORM = 2, FIELD_UV_THETA = 3 };
 hipError_t launch_local_fields(int fp64, int ncomp, const Geometry &g, const void *psi, int kind, double *out,

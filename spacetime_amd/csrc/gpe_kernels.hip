@@ -708,6 +708,28 @@ __global__ __launch_bounds__(256) void k_observables(ObsArgs a)
         acc[OBS_RX] = n * rx * dv;
         acc[OBS_RY] = n * ry * dv;
         acc[OBS_RZ] = n * rz * dv;
+        if constexpr (C > 1) {
+            // the reference's local spin (localAvgSpinAndDensity, Spin1GpeBCC.cu:235-271; Spin2GpeBCC.cu
+            // 214-262 applies the same spin-1 expressions to m = 1, 0, -1): x, y of the spinor divided
+            // by sqrt(n), z = |psi_1|^2 - |psi_-1|^2 undivided; its integrals weight it by n
+            // (integrateSpinAndDensity, Spin1GpeBCC.cu:992-1020).  Unknowns with n = 0 add nothing.
+            constexpr int o = C == 5 ? 1 : 0;
+            const double zr = (pr[o] * pr[o] + pi[o] * pi[o]) - (pr[o + 2] * pr[o + 2] + pi[o + 2] * pi[o + 2]);
+            double sx = 0, sy = 0;
+            if (n > 0) {
+                const double sq = sqrt(n);
+                double ur[3], ui[3];
+                for (int k = 0; k < 3; ++k) {
+                    ur[k] = pr[o + k] / sq;
+                    ui[k] = pi[o + k] / sq;
+                }
+                // sqrt2 (conj(u1) u0 + conj(u0) u-1)
+                sx = 1.4142135623730951 * ((ur[0] * ur[1] + ui[0] * ui[1]) + (ur[1] * ur[2] + ui[1] * ui[2]));
+                sy = 1.4142135623730951 * ((ur[0] * ui[1] - ui[0] * ur[1]) + (ur[1] * ui[2] - ui[1] * ur[2]));
+            }
+            acc[OBS_REF_SPIN] = n * sqrt(sx * sx + sy * sy + zr * zr) * dv;
+            acc[OBS_REF_MZ] = n * zr * dv;
+        }
     }
     __shared__ double red[4][OBS_COUNT];
 #pragma unroll

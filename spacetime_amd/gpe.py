@@ -61,6 +61,7 @@ class _Obs(ctypes.Structure):
         ("norm", ctypes.c_double), ("energy", ctypes.c_double), ("expect_h", ctypes.c_double),
         ("magnetization", ctypes.c_double * 3), ("max_density", ctypes.c_double), ("time", ctypes.c_double),
         ("finite", ctypes.c_int), ("spin", ctypes.c_double), ("moment", ctypes.c_double * 3),
+        ("ref_spin", ctypes.c_double), ("ref_magnetization_z", ctypes.c_double),
     ]
 
 
@@ -155,6 +156,10 @@ def load_library(path=LIB_PATH):
     lib.gpe_host_write_vtk_preimage.argtypes = [cp, vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_double]
     lib.gpe_host_write_vtk_volume.argtypes = [cp, vp, ctypes.c_char_p, ctypes.c_int]
     lib.gpe_write_bmp.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
+    lib.gpe_series_create.argtypes = [ctypes.POINTER(vp)]
+    lib.gpe_series_destroy.argtypes = [vp]
+    lib.gpe_series_append.argtypes = [vp, vp, ctypes.c_int]
+    lib.gpe_series_save.argtypes = [vp, ctypes.c_char_p]
     lib.gpe_host_write_bmp.argtypes = [cp, vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
     _LIB = lib
     return lib
@@ -229,6 +234,8 @@ class Observables:
     finite: bool
     spin: float = 0.0
     moment: np.ndarray = None
+    ref_spin: float = 0.0
+    ref_magnetization_z: float = 0.0
 
 
 class Stepper:
@@ -436,7 +443,7 @@ class Stepper:
         if check or rc not in (0, -34):
             self._check(rc, "gpe_observables")
         return Observables(o.norm, o.energy, o.expect_h, np.array(o.magnetization[:]), o.max_density, o.time,
-                           bool(o.finite), o.spin, np.array(o.moment[:]))
+                           bool(o.finite), o.spin, np.array(o.moment[:]), o.ref_spin, o.ref_magnetization_z)
 
     def max_hamiltonian(self, level=0):
         """Maximum local Hamiltonian bound of a level (gpe_max_hamiltonian; scalar and spin-1)."""
@@ -556,3 +563,38 @@ class Ensemble:
             self.close()
         except Exception:
             pass
+
+
+class Series:
+    """The reference's time-series (.m) file of a run (gpe_series_*): append(stepper, level) adds the
+    level's time, field and spin integrals as one column; save(path) writes the eight MATLAB
+    vectors and starts a new series."""
+
+    def __init__(self):
+        self.lib = load_library()
+        h = ctypes.c_void_p()
+        rc = self.lib.gpe_series_create(ctypes.byref(h))
+        if rc != 0:
+            raise GpeError("gpe_series_create failed: %s" % ERRORS.get(rc, rc))
+        self._h = h
+
+    def append(self, stepper, level=0):
+        rc = self.lib.gpe_series_append(self._h, stepper._h, level)
+        if rc != 0:
+            raise GpeError("gpe_series_append failed: %s" % ERRORS.get(rc, rc))
+
+    def save(self, path):
+        rc = self.lib.gpe_series_save(self._h, os.fsencode(str(path)))
+        if rc != 0:
+            raise GpeError("gpe_series_save failed: %s" % ERRORS.get(rc, rc))
+
+    def close(self):
+        if self._h:
+            self.lib.gpe_series_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

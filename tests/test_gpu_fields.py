@@ -1,6 +1,7 @@
 """Per-unknown field maps (gpe_local_fields) and the spin / first-moment observables against
 textbook spin algebra on random states (dense F matrices from the angular-momentum ladder),
-plus closed forms of the spin-1 (u, v, theta) decomposition on polar and ferromagnetic
+the reference's integrated "Spin" and mag_z against its formulas and closed forms on uniform
+states, plus closed forms of the spin-1 (u, v, theta) decomposition on polar and ferromagnetic
 spinors.  FP64 contexts: 1e-12 relative."""
 import numpy as np
 import pytest
@@ -61,6 +62,48 @@ def test_density_spin_maps_and_integrals(ncomp, lattice):
                                atol=1e-12 * np.sum(n * dv))
     np.testing.assert_allclose(obs.magnetization, np.sum(Fv * dv[:, None], axis=(0, 1, 2, 3)), rtol=1e-12,
                                atol=1e-12 * np.sum(n * dv))
+    # the reference's "Spin" and mag_z (localAvgSpinAndDensity + integrateSpinAndDensity): x, y of the
+    # spinor divided by sqrt(n) on m = 1, 0, -1, z = |psi_1|^2 - |psi_-1|^2 undivided, weighted by n
+    ref_spin = ref_mz = 0.0
+    if ncomp > 1:
+        o = 1 if ncomp == 5 else 0
+        p1, p0, pm = psi[o], psi[o + 1], psi[o + 2]
+        with np.errstate(invalid="ignore", divide="ignore"):
+            u1, u0, um = (np.where(n > 0, q / np.sqrt(n), 0.0) for q in (p1, p0, pm))
+        t = np.sqrt(2.0) * (u1.conj() * u0 + u0.conj() * um)
+        zr = np.abs(p1) ** 2 - np.abs(pm) ** 2
+        ref_spin = np.sum(n * np.sqrt(t.real ** 2 + t.imag ** 2 + zr ** 2) * dv)
+        ref_mz = np.sum(n * zr * dv)
+    assert obs.ref_spin == pytest.approx(ref_spin, rel=1e-12, abs=1e-14)
+    assert obs.ref_magnetization_z == pytest.approx(ref_mz, rel=1e-12, abs=1e-12 * np.sum(n * n * dv))
+
+
+@pytest.mark.parametrize("name,spinor,ref_spin_per_n2,ref_mz_per_n2", [
+    ("ferro", (0.8, 0.0, 0.0), 1.0, 1.0),                 # |s| = |z| = n: both weighted by n once more
+    ("polar", (0.0, 0.8, 0.0), 0.0, 0.0),
+    ("ferro_x", (0.4, 0.8 / np.sqrt(2.0), 0.4), None, 0.0),  # transverse: |s| = 1, the physical value
+])
+def test_reference_spin_integrals_closed_forms(name, spinor, ref_spin_per_n2, ref_mz_per_n2):
+    """Uniform spin-1 states: the reference's integrated z spin is sum dv n (|psi_1|^2 - |psi_-1|^2)
+    = n^2 V for a ferromagnetic state, where the physical magnetisation is n V; a transverse
+    ferromagnetic state gives its "Spin" as n V, as the physical one."""
+    from spacetime_amd import states
+    cfg = _cfg(3)
+    psi = np.zeros(cfg.shape, complex)
+    psi[:] = (np.exp(0.3j) * np.array(spinor))[:, None, None, None, None]
+    n = float(np.sum(np.abs(np.array(spinor)) ** 2))
+    vol = float(np.sum(states.dual_volumes(cfg) * np.ones(cfg.shape[1:])))
+    with gpe.Stepper(cfg) as st:
+        st.set_state(psi)
+        obs = st.observables(0)
+    if ref_spin_per_n2 is None:
+        assert obs.ref_spin == pytest.approx(n * vol, rel=1e-12)
+        assert obs.spin == pytest.approx(n * vol, rel=1e-12)
+    else:
+        assert obs.ref_spin == pytest.approx(ref_spin_per_n2 * n * n * vol, rel=1e-12, abs=1e-14)
+    assert obs.ref_magnetization_z == pytest.approx(ref_mz_per_n2 * n * n * vol, rel=1e-12, abs=1e-14)
+    if name == "ferro":
+        assert obs.magnetization[2] == pytest.approx(n * vol, rel=1e-12)
 
 
 def test_uv_theta_spin1():

@@ -196,6 +196,57 @@ def test_pictures_from_a_context_match_the_host_writer(tmp_path, ncomp, styles):
                 assert rgb.max() > 0
 
 
+def test_series_file_in_the_reference_format(tmp_path):
+    """The .m time series (Spin1GpeBCC.cu:1497-1504,1659-1712): eight lines "name = [name, v, ...];"
+    holding, per append, the time on the ramp's clock, Bq, Bb x + y + z, and the reference's
+    Spin, mag_x, mag_y, mag_z and norm of the level; a save starts a new series."""
+    import re
+    cfg = _cfg(gpe.FP64)
+    psi = normalized(random_state(cfg.shape, 31), cfg)
+    ramp_scale = 7.5
+    q = [(0.3, 0.001, 0), (0.6, 0.1, 1)]
+    b = [((0.01, -0.02, 0.05), 0.002, 0), ((0.03, 0.01, -0.04), 0.1, 1)]
+    with gpe.Stepper(cfg) as st, gpe.Series() as ser:
+        st.set_ramp(q_segments=q, b_segments=b, time_scale=ramp_scale)
+        st.set_state(psi)
+        st.start(gpe.START_EULER)
+        want = []
+        for k in range(3):
+            st.step(5)
+            o = st.observables(1)
+            ser.append(st, 1)
+            want.append(o)
+        ser.save(tmp_path / "a.m")
+        st.step(2)
+        ser.append(st, 0)
+        o_last = st.observables(0)
+        ser.save(tmp_path / "b.m")
+    lines = (tmp_path / "a.m").read_text().splitlines()
+    names = ["times", "Bq", "Bz", "Spin", "mag_x", "mag_y", "mag_z", "norm"]
+    assert [ln.split(" = ")[0] for ln in lines] == names
+    cols = {}
+    for name, ln in zip(names, lines):
+        m = re.fullmatch(r"%s = \[%s((?:, -?\d+\.\d{18})*)\];" % (name, name), ln)
+        assert m, ln
+        cols[name] = [float(v) for v in m.group(1).split(", ")[1:]]
+        assert len(cols[name]) == 3
+    for k, o in enumerate(want):
+        assert cols["times"][k] == pytest.approx(o.time * ramp_scale, abs=1e-15)
+        assert cols["Spin"][k] == pytest.approx(o.ref_spin, rel=1e-15, abs=1e-17)
+        assert cols["mag_x"][k] == pytest.approx(o.magnetization[0], abs=1e-17)
+        assert cols["mag_y"][k] == pytest.approx(o.magnetization[1], abs=1e-17)
+        assert cols["mag_z"][k] == pytest.approx(o.ref_magnetization_z, abs=1e-17)
+        assert cols["norm"][k] == pytest.approx(o.norm, rel=1e-15)
+    # the field columns: the ramps at those times, inside their second (linear) segments
+    for k, tr in enumerate(cols["times"]):
+        assert 0.002 < tr < 0.1
+        assert cols["Bq"][k] == pytest.approx(0.3 + 0.3 * (tr - 0.001) / 0.1, rel=1e-12)
+        assert cols["Bz"][k] == pytest.approx(0.04 - 0.04 * (tr - 0.002) / 0.1, rel=1e-12)
+    b_lines = (tmp_path / "b.m").read_text().splitlines()
+    assert b_lines[0].startswith("times = [times, ") and b_lines[0].count(",") == 1  # a new series
+    assert float(b_lines[7][len("norm = [norm, "):-2]) == pytest.approx(o_last.norm, rel=1e-15)
+
+
 def test_fp64_trajectory_adjoint_of_an_fp32_context():
     """fp64_trajectory on an FP32 context: the same J and gradients as an FP64 context's
     adjoint started from the state the FP32 context holds (psi rounded to FP32), and the
