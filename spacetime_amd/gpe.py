@@ -593,6 +593,12 @@ class Series:
             self.lib.gpe_series_destroy(self._h)
             self._h = None
 
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
     def __enter__(self):
         return self
 
