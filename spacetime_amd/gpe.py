@@ -589,7 +589,7 @@ class Series:
             raise GpeError("gpe_series_save failed: %s" % ERRORS.get(rc, rc))
 
     def close(self):
-        if self._h:
+        if getattr(self, "_h", None):
             self.lib.gpe_series_destroy(self._h)
             self._h = None
 
