@@ -124,21 +124,6 @@ typedef struct {
                               mag_x, mag_y are magnetization[0], [1] */
 } gpe_observables_t;
 
-/* The reference's time-series file of a run (Spin1GpeBCC.cu:1497-1504, 1659-1712): each append
- * adds one column -- the level's time times the ramp's time scale (the reference's ms clock; 1
- * without a ramp), the field's Bq and Bb x + y + z at that time, ref_spin, magnetization[0],
- * magnetization[1], ref_magnetization_z and norm -- and each save writes
- *   times = [times, t1, t2, ...];  Bq = [Bq, ...];  Bz = [Bz, ...];  Spin = [Spin, ...];
- *   mag_x = [mag_x, ...];  mag_y = [...];  mag_z = [...];  norm = [norm, ...];
- * one per line, values as "%.18f" (the reference's toString), then starts a new series, so
- * running the saved files in order concatenates them as the reference's do.  append returns
- * GPE_ERANGE (after appending) for a non-finite state. */
-typedef struct gpe_series gpe_series;
-int gpe_series_create(gpe_series **out);
-int gpe_series_destroy(gpe_series *series);
-int gpe_series_append(gpe_series *series, gpe_ctx *ctx, int level);
-int gpe_series_save(gpe_series *series, const char *path);
-
 /* Create a context: allocates both time levels on the current HIP device and uploads
  * the lattice tables.  Replaces integrateInTime's set-up (Spin1GpeBCC.cu:1105-1172). */
 int gpe_create(const gpe_config *cfg, gpe_ctx **out);
@@ -186,6 +171,21 @@ int gpe_itp(gpe_ctx *ctx, int nsteps, double dt_itp);
 /* FP64 observables of a level.  Replaces density/innerProduct/localAvgSpinAndDensity and
  * the integrate trees (Spin1GpeBCC.cu:191-271,328-392,955-1020). */
 int gpe_observables(gpe_ctx *ctx, int level, gpe_observables_t *out);
+
+/* The reference's time-series file of a run (Spin1GpeBCC.cu:1497-1504, 1659-1712): each append
+ * adds one column -- the level's time times the ramp's time scale (the reference's ms clock; 1
+ * without a ramp), the field's Bq and Bb x + y + z at that time, ref_spin, magnetization[0],
+ * magnetization[1], ref_magnetization_z and norm -- and each save writes
+ *   times = [times, t1, t2, ...];  Bq = [Bq, ...];  Bz = [Bz, ...];  Spin = [Spin, ...];
+ *   mag_x = [mag_x, ...];  mag_y = [...];  mag_z = [...];  norm = [norm, ...];
+ * one per line, values as "%.18f" (the reference's toString), then starts a new series, so
+ * running the saved files in order concatenates them as the reference's do.  append returns
+ * GPE_ERANGE (after appending) for a non-finite state. */
+typedef struct gpe_series gpe_series;
+int gpe_series_create(gpe_series **out);
+int gpe_series_destroy(gpe_series *series);
+int gpe_series_append(gpe_series *series, gpe_ctx *ctx, int level);
+int gpe_series_save(gpe_series *series, const char *path);
 
 /* Maximum over the unknowns of a level of the local Hamiltonian bound: with p = V(r) + c0 n
  * and B = B(r, t) + c1 psi^dag F psi, max(p + |Bx|/sqrt2, p + |By|/sqrt2, p + |Bz|) for spin-1,
