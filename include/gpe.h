@@ -249,26 +249,29 @@ int gpe_ensemble_timing(const gpe_ensemble *ens, double *ms_per_sweep, double *u
 
 /* -------------------------------------------------------------------- autotuning ---- */
 /* Time the traversal candidates (y-tile rows 1..32, and 3..28 where they give row groups that
- * are not powers of two, x XCD remap on/off x one or two x values per lane, the latter for
- * FP32) on scratch copies of the context's
- * two levels for trial_sweeps sweeps each, and keep the fastest.  Results are cached per
+ * are not powers of two, x the order flags of gpe_set_traversal -- the second only where lines
+ * span several waves -- x one or two x values per lane, the latter for FP32) on scratch copies
+ * of the context's two levels for trial_sweeps sweeps each, and keep the fastest.  Results are cached per
  * (lattice, ncomp, precision, nx, ny, nz) for the process; pass trial_sweeps = 0 to apply a
  * cached choice only.  Outputs may be NULL.  No counterpart in the reference, which fixes its
  * 16 x 2 x 1 thread blocks at compile time (Spin1GpeBCC.cu:39-41). */
 int gpe_autotune(gpe_ctx *ctx, int trial_sweeps, int *tile_rows, int *xcd_remap, int *x_per_lane,
                  double *ms_per_sweep);
 
-/* Set / read the sweep traversal directly: y-tile rows (1..64), XCD remap (0/1), x values per
- * lane (1, or 2 for FP32).  No counterpart in the reference (fixed thread blocks,
- * Spin1GpeBCC.cu:39-41).  Results do not depend on it beyond
- * FP32 instruction selection. */
+/* Set / read the sweep traversal directly: y-tile rows (1..64), order flags (0..3: bit 0 gives
+ * each XCD a contiguous range of the traversal, bit 1 puts the unknown v inside the x segment
+ * so that a workgroup's waves take neighbouring unknowns of one stretch of x; the second
+ * matters only where lines span several waves), x values per lane (1, or 2 for FP32).  No
+ * counterpart in the reference (fixed thread blocks, Spin1GpeBCC.cu:39-41).  Results do not
+ * depend on it beyond FP32 instruction selection and the order of FP64 partial sums. */
 int gpe_set_traversal(gpe_ctx *ctx, int tile_rows, int xcd_remap, int x_per_lane);
 int gpe_traversal(const gpe_ctx *ctx, int *tile_rows, int *xcd_remap, int *x_per_lane);
 
 /* The launch a sweep of a configuration would make for a traversal, computed on the host
  * without a device: workgroups of four waves, each wave a 64-lane segment of one (z, y, v)
  * x-line (rows = 1) or of a group of `rows` y rows (x pairs spread over the rows), in the
- * order y-tile, z, row (group) in tile, unknown v, segment; and the constants of the wave
+ * order y-tile, z, row (group) in tile, unknown v, segment (or segment, v: flag bit 1; the
+ * plan's xcd_remap holds the flags); and the constants of the wave
  * decode's divisions by invariants (d, m, l: n / d = (t + n) >> l for n < 2^31, t the high
  * 32 bits of n m).  No counterpart in the reference, whose
  * launch shape is fixed (Spin1GpeBCC.cu:1376-1380). */

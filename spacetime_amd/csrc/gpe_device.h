@@ -81,18 +81,36 @@ __device__ __forceinline__ unsigned fast_div(unsigned n, const FastDiv &f)
     return (__umulhi(n, f.m) + n) >> f.l;
 }
 
+// The two fastest traversal indices of wave w: the unknown v and the x segment, the segment
+// fastest, or (g.v_inner) v fastest so that a workgroup's waves take neighbouring unknowns of
+// one segment.  Returns w / (V nseg).
+__device__ __forceinline__ unsigned split_v_seg(const Geometry &g, unsigned w, int *v, int *xseg)
+{
+    if (g.v_inner) {
+        const unsigned q0 = fast_div(w, g.f_V);
+        *v = (int)(w - q0 * (unsigned)g.V);
+        const unsigned q1 = fast_div(q0, g.f_nseg);
+        *xseg = (int)(q0 - q1 * (unsigned)g.nseg);
+        return q1;
+    }
+    const unsigned q0 = fast_div(w, g.f_nseg);
+    *xseg = (int)(w - q0 * (unsigned)g.nseg);
+    const unsigned q1 = fast_div(q0, g.f_V);
+    *v = (int)(q0 - q1 * (unsigned)g.V);
+    return q1;
+}
+
 // Wave coordinates of workgroup b of nb (b, nb: the workgroup's place in its own run) for a
 // traversal with one y row per wave (g.rows == 1).
-// Traversal order, slowest to fastest: y-tile, z, row in tile, unknown v, x-segment.
+// Traversal order, slowest to fastest: y-tile, z, row in tile, unknown v, x-segment (or x-segment,
+// unknown v: split_v_seg).
 __device__ __forceinline__ WaveCoord decode(const Geometry &g, int b, int nb)
 {
     WaveCoord wc;
     const unsigned wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const unsigned w = (unsigned)(g.xcd_remap ? logical_block(b, nb) : b) * 4u + wid;
-    const unsigned q0 = fast_div(w, g.f_nseg);
-    const int xseg = (int)(w - q0 * (unsigned)g.nseg);
-    const unsigned q1 = fast_div(q0, g.f_V);
-    wc.v = (int)(q0 - q1 * (unsigned)g.V);
+    int xseg;
+    const unsigned q1 = split_v_seg(g, w, &wc.v, &xseg);
     const unsigned q2 = fast_div(q1, g.f_ty);
     const int yin = (int)(q1 - q2 * (unsigned)g.ty);
     const unsigned q3 = fast_div(q2, g.f_nz);
@@ -120,10 +138,8 @@ __device__ __forceinline__ RowCoord decode_rows(const Geometry &g, int b, int nb
     RowCoord rc;
     const unsigned wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const unsigned w = (unsigned)(g.xcd_remap ? logical_block(b, nb) : b) * 4u + wid;
-    const unsigned q0 = fast_div(w, g.f_nseg);
-    const int xseg = (int)(w - q0 * (unsigned)g.nseg);
-    const unsigned q1 = fast_div(q0, g.f_V);
-    rc.v = (int)(q0 - q1 * (unsigned)g.V);
+    int xseg;
+    const unsigned q1 = split_v_seg(g, w, &rc.v, &xseg);
     const unsigned q2 = fast_div(q1, g.f_ty);
     const int yg = (int)(q1 - q2 * (unsigned)g.tyr);
     const unsigned q3 = fast_div(q2, g.f_nz);

@@ -1528,7 +1528,7 @@ int gpe_ensemble_step(gpe_ensemble *e, int nsteps, int graph_sweeps)
         if (!c->level1_valid) return ens_fail(e, GPE_ESTATE, "member not started");
         const Geometry &t = e->trav[i];
         if (c->gs.blocks != t.blocks || c->gs.xpl != t.xpl || c->gs.rows != t.rows || c->gs.ty != t.ty ||
-            c->gs.xcd_remap != t.xcd_remap)
+            c->gs.xcd_remap != t.xcd_remap || c->gs.v_inner != t.v_inner)
             return ens_fail(e, GPE_ESTATE, "member traversal changed after the ensemble was created");
         int rc = upload_fields(c, c->sweeps, nsw);
         if (rc) return ens_fail(e, rc, c->err);
@@ -1614,14 +1614,16 @@ int gpe_ensemble_timing(const gpe_ensemble *e, double *ms_per_sweep, double *unk
 // ------------------------------------------------------------------------ autotuning ---
 
 // Apply a traversal to both geometries; on failure (a workgroup count past the 2^29 limit of
-// the decode) both are left as they were and false is returned.
-static bool set_traversal(gpe_ctx *c, int ty, int remap, int xpl)
+// the decode) both are left as they were and false is returned.  flags: bit 0 the XCD remap,
+// bit 1 the unknown v inside the x segment in the traversal order.
+static bool set_traversal(gpe_ctx *c, int ty, int flags, int xpl)
 {
     Geometry g = c->g, gs = c->gs;
     gs.xpl = xpl;
     for (Geometry *p : {&g, &gs}) {
         p->ty = ty;
-        p->xcd_remap = remap;
+        p->xcd_remap = flags & 1;
+        p->v_inner = (flags >> 1) & 1;
         if (!finish_geometry(*p)) return false;
     }
     c->g = g;
@@ -1644,7 +1646,7 @@ static int grow_partials(gpe_ctx *c, int blocks_before)
 int gpe_set_traversal(gpe_ctx *c, int tile_rows, int xcd_remap, int x_per_lane)
 {
     if (!c) return GPE_EINVAL;
-    if (tile_rows < 1 || tile_rows > 64 || (xcd_remap != 0 && xcd_remap != 1) || (x_per_lane != 1 && x_per_lane != 2))
+    if (tile_rows < 1 || tile_rows > 64 || xcd_remap < 0 || xcd_remap > 3 || (x_per_lane != 1 && x_per_lane != 2))
         return fail(c, GPE_EINVAL, "bad traversal");
     if (x_per_lane == 2 && !pair_lanes_ok(c->fp64, c->g.deg, c->cfg.ncomp))
         return fail(c, GPE_EINVAL, "two x per lane needs FP32");
@@ -1658,7 +1660,7 @@ int gpe_traversal(const gpe_ctx *c, int *tile_rows, int *xcd_remap, int *x_per_l
 {
     if (!c) return GPE_EINVAL;
     if (tile_rows) *tile_rows = c->gs.ty;
-    if (xcd_remap) *xcd_remap = c->gs.xcd_remap;
+    if (xcd_remap) *xcd_remap = c->gs.xcd_remap | (c->gs.v_inner << 1);
     if (x_per_lane) *x_per_lane = c->gs.xpl;
     return GPE_OK;
 }
@@ -1669,12 +1671,13 @@ int gpe_plan_traversal(const gpe_config *cfg, int tile_rows, int xcd_remap, int 
     if (cfg->lattice < 0 || cfg->lattice > 5 || (cfg->ncomp != 1 && cfg->ncomp != 3 && cfg->ncomp != 5)) return GPE_EINVAL;
     if (cfg->precision != GPE_FP32 && cfg->precision != GPE_FP64) return GPE_EINVAL;
     if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1) return GPE_EINVAL;
-    if (tile_rows < 1 || tile_rows > 64 || (xcd_remap != 0 && xcd_remap != 1) || (x_per_lane != 1 && x_per_lane != 2))
+    if (tile_rows < 1 || tile_rows > 64 || xcd_remap < 0 || xcd_remap > 3 || (x_per_lane != 1 && x_per_lane != 2))
         return GPE_EINVAL;
     Geometry g = base_geometry(*cfg);
     if (x_per_lane == 2 && !pair_lanes_ok(cfg->precision == GPE_FP64, g.deg, cfg->ncomp)) return GPE_EINVAL;
     g.ty = tile_rows;
-    g.xcd_remap = xcd_remap;
+    g.xcd_remap = xcd_remap & 1;
+    g.v_inner = (xcd_remap >> 1) & 1;
     g.xpl = x_per_lane;
     if (!finish_geometry(g)) return GPE_EINVAL;
     out->nx = g.nx;
@@ -1689,7 +1692,7 @@ int gpe_plan_traversal(const gpe_config *cfg, int tile_rows, int xcd_remap, int 
     out->tile_rows = g.ty;
     out->row_groups = g.tyr;
     out->ntile = g.ntile;
-    out->xcd_remap = g.xcd_remap;
+    out->xcd_remap = g.xcd_remap | (g.v_inner << 1);
     out->lx = g.lx;
     out->waves = g.waves;
     out->blocks = g.blocks;
@@ -1713,7 +1716,7 @@ int gpe_autotune(gpe_ctx *c, int trial_sweeps, int *tile_rows, int *xcd_remap, i
     if (trial_sweeps < 0) return fail(c, GPE_EINVAL, "negative trial sweeps");
     const TuneKey key(c->cfg.lattice, c->cfg.ncomp, c->fp64, c->cfg.nx, c->cfg.ny, c->cfg.nz);
     const int max_blocks_before = c->g.blocks;
-    int best_ty = c->g.ty, best_remap = c->g.xcd_remap, best_xpl = c->gs.xpl;
+    int best_ty = c->g.ty, best_remap = c->gs.xcd_remap | (c->gs.v_inner << 1), best_xpl = c->gs.xpl;
     double best_ms = -1.0;
     {
         std::lock_guard<std::mutex> lk(g_tune_mutex);
@@ -1754,9 +1757,11 @@ int gpe_autotune(gpe_ctx *c, int trial_sweeps, int *tile_rows, int *xcd_remap, i
         for (int ty : tys) {
             if (ty > c->g.ny && ty != 1) continue;
             const bool pow2 = (ty & (ty - 1)) == 0;
-            for (int remap = 0; remap < 2; ++remap) {
+            // flags: XCD remap, and the unknown inside the x segment (only where lines span several waves)
+            for (int remap = 0; remap < 4; ++remap) {
                 if (!set_traversal(c, ty, remap, xpl)) continue;  // too many workgroups: not a candidate
                 if (!pow2 && (c->gs.rows & (c->gs.rows - 1)) == 0) continue;
+                if ((remap & 2) && c->gs.nseg == 1) continue;
                 void *cur = trial.s0, *nxt = trial.s1;
                 for (int k = 0; k < trial_sweeps + 2; ++k) {
                     if (k == 2) HIPCALL(c, hipEventRecord(c->ev0, c->stream));

@@ -46,6 +46,7 @@ struct Geometry {
     long long waves;   // total waves of a sweep
     int blocks;        // workgroups of a sweep (4 waves each)
     int xcd_remap;     // give each XCD a contiguous range of the traversal order
+    int v_inner;       // 1: the unknown v varies faster than the x segment (lines of several waves)
 };
 
 // x values per lane of the sweep kernel.  Two (one 16-byte load per component and neighbour,

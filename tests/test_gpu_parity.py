@@ -70,14 +70,17 @@ def test_fp32_sweeps_match_oracle(oracle, lattice_name, ncomp, xpl):
     assert rel_l2(g1, o1) < 2e-5
 
 
-@pytest.mark.parametrize("precision,tol,xpl,nx", [(1, 1e-12, 1, 70), (0, 2e-5, 1, 70), (0, 2e-5, 2, 70),
-                                                  (0, 2e-5, 2, 129), (0, 2e-5, 2, 128), (0, 2e-5, 1, 129)])
-def test_wide_rows_span_several_waves(oracle, precision, tol, xpl, nx):
+@pytest.mark.parametrize("precision,tol,xpl,nx,flags", [
+    (1, 1e-12, 1, 70, 1), (0, 2e-5, 1, 70, 1), (0, 2e-5, 2, 70, 1), (0, 2e-5, 2, 129, 1), (0, 2e-5, 2, 128, 1),
+    (0, 2e-5, 1, 129, 1),
+    # the unknown inside the x segment in the traversal order, with and without the XCD remap
+    (1, 1e-12, 1, 130, 3), (1, 1e-12, 1, 70, 2), (0, 2e-5, 2, 300, 3), (0, 2e-5, 1, 129, 2)])
+def test_wide_rows_span_several_waves(oracle, precision, tol, xpl, nx, flags):
     """nx > 64 (or > 128 with x pairs) exercises several wave segments per x-line, partial last
     waves, the line-end pair clamp (nx = 128: the last pair's +x neighbour pair starts at lx - 1)
-    and a lone last value (odd nx)."""
+    and a lone last value (odd nx); both orders of unknowns and segments."""
     cfg = _config("bcc", 3, precision, (nx, 3, 2))
-    (g0, g1), (o0, o1) = _run_both(oracle, cfg, 6, field=FIELD, traversal=(2, 1, xpl))
+    (g0, g1), (o0, o1) = _run_both(oracle, cfg, 6, field=FIELD, traversal=(2, flags, xpl))
     assert rel_l2(g0, o0) < tol and rel_l2(g1, o1) < tol
 
 
@@ -100,6 +103,10 @@ def test_wide_rows_span_several_waves(oracle, precision, tol, xpl, nx):
     ("z", 3, (68, 16, 2), (14, 1, 2)),   # 7-row groups of 34 pairs over 4 waves; tile 2 holds 2 rows
     ("bcc", 1, (70, 9, 2), (7, 0, 2)),   # 7-row groups of 35 pairs, no XCD remap
     ("c15", 3, (20, 7, 2), (6, 1, 2)),   # 6-row groups of 10 pairs in one wave
+    # the unknown inside the x segment: row groups over several waves
+    ("bcc", 1, (192, 6, 3), (4, 3, 2)),  # 2-row groups of 96 pairs over 3 waves; tile 2 holds rows 4, 5
+    ("a15", 5, (82, 7, 2), (6, 2, 2)),   # 3-row groups of 41 pairs over 2 waves, no XCD remap
+    ("z", 3, (68, 16, 2), (14, 3, 2)),   # 7-row groups of 34 pairs over 4 waves
 ])
 def test_packed_rows_match_oracle(oracle, lattice_name, ncomp, dims, traversal):
     """x pairs spread over groups of 2-8 y rows (lines that one row per wave leaves partly idle)."""
@@ -118,8 +125,12 @@ def test_traversal_setter_validates():
             st.set_traversal(0, 1, 1)
         with pytest.raises(gpe.GpeError, match="EINVAL"):
             st.set_traversal(8, 1, 4)  # one or two x per lane only
+        with pytest.raises(gpe.GpeError, match="EINVAL"):
+            st.set_traversal(8, 4, 1)  # order flags 0..3
         st.set_traversal(4, 0, 1)
         assert st.traversal() == dict(tile_rows=4, xcd_remap=0, x_per_lane=1)
+        st.set_traversal(4, 2, 1)
+        assert st.traversal() == dict(tile_rows=4, xcd_remap=2, x_per_lane=1)
 
 
 def test_loss_term(oracle):

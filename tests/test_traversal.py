@@ -79,21 +79,28 @@ def _decode_all(p):
     """Every (workgroup, wave, lane) of a sweep: arrays of v, z, y, x, live flag."""
     nb = p.blocks
     b = np.arange(nb, dtype=np.int64)
-    lb = _logical_block(b, nb) if p.xcd_remap else b
+    lb = _logical_block(b, nb) if p.xcd_remap & 1 else b
     w = (lb[:, None] * 4 + np.arange(4)[None, :]).reshape(-1).astype(np.uint64)
     assert int(w.max()) < 2 ** 31
-    q0 = _fdiv(w, p.div_nseg)
-    xseg = (w - q0 * np.uint64(p.nseg)).astype(np.int64)
-    q1 = _fdiv(q0, p.div_V)
-    v = (q0 - q1 * np.uint64(p.V)).astype(np.int64)
+    wi = w.astype(np.int64)
+    if p.xcd_remap & 2:  # the unknown inside the x segment
+        q0 = _fdiv(w, p.div_V)
+        v = (w - q0 * np.uint64(p.V)).astype(np.int64)
+        q1 = _fdiv(q0, p.div_nseg)
+        xseg = (q0 - q1 * np.uint64(p.nseg)).astype(np.int64)
+        assert np.array_equal(q0.astype(np.int64), wi // p.V)
+    else:
+        q0 = _fdiv(w, p.div_nseg)
+        xseg = (w - q0 * np.uint64(p.nseg)).astype(np.int64)
+        q1 = _fdiv(q0, p.div_V)
+        v = (q0 - q1 * np.uint64(p.V)).astype(np.int64)
+        assert np.array_equal(q0.astype(np.int64), wi // p.nseg)
     q2 = _fdiv(q1, p.div_rows)
     q3 = _fdiv(q2, p.div_nz)
     z = (q2 - q3 * np.uint64(p.nz)).astype(np.int64)
     # exact divisions for comparison
-    wi = w.astype(np.int64)
-    assert np.array_equal(q0.astype(np.int64), wi // p.nseg)
-    assert np.array_equal(q1.astype(np.int64), (wi // p.nseg) // p.V)
-    assert np.array_equal(q2.astype(np.int64), ((wi // p.nseg) // p.V) // p.row_groups)
+    assert np.array_equal(q1.astype(np.int64), wi // (p.nseg * p.V))
+    assert np.array_equal(q2.astype(np.int64), (wi // (p.nseg * p.V)) // p.row_groups)
     lane = np.arange(64, dtype=np.int64)
     if p.rows == 1:
         yin = (q1 - q2 * np.uint64(p.tile_rows)).astype(np.int64)
@@ -135,7 +142,7 @@ def test_decode_stays_in_range_and_covers_the_box(lat, ncomp, precision, dims):
     checked = 0
     for xpl in xpls:
         for ty in (1, 2, 3, 4, 5, 6, 7, 8, 12, 14, 16, 32, 64):
-            for remap in (0, 1):
+            for remap in (0, 1, 2, 3):
                 rc, p = _plan(cfg, ty, remap, xpl)
                 assert rc == 0
                 assert p.blocks * 4 >= p.waves and p.tile_rows <= ny and p.tile_rows % p.rows == 0
@@ -168,7 +175,7 @@ def test_decode_stays_in_range_and_covers_the_box(lat, ncomp, precision, dims):
                     assert np.all(x[live] + 1 < p.lx)
                 assert count.min() == 1 and count.max() == 1, (ty, remap, xpl)
                 checked += 1
-    assert checked >= 16
+    assert checked >= 32
 
 
 @pytest.mark.parametrize("lat,ncomp,dims,ty,rows", [
@@ -218,3 +225,6 @@ def test_plan_rejects_bad_arguments():
     assert _plan(gpe.Config(lattice="bcc", ncomp=1, dims=(8, 8, 8)), 8, 1, 4)[0] == -22
     assert _plan(cfg, 0, 1, 1)[0] == -22
     assert _plan(gpe.Config(lattice="bcc", dims=(0, 8, 8)), 8, 1, 1)[0] == -22
+    assert _plan(cfg, 8, 4, 1)[0] == -22 and _plan(cfg, 8, -1, 1)[0] == -22  # order flags 0..3
+    rc, p = _plan(cfg, 8, 3, 1)
+    assert rc == 0 and p.xcd_remap == 3
