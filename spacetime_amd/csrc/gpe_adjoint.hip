@@ -414,13 +414,16 @@ __device__ __forceinline__ void adjoint_pair(const AdjArgs &a, int b, int nb, do
     if (!valid) return;
     bool rowok = true;
     int y = y0, roff = 0;
-    const int rowstride = g.V * g.C * (int)g.lx;
+    const int rowstride = (int)g.ys;
+    int ylo = y0, yhi = y0;  // the rows of the wave's lanes (a lane's row grows with its lane number)
     if constexpr (ROWS) {
         const int nrow = g.ny - y0 < g.rows ? g.ny - y0 : g.rows;
         rowok = r < nrow;
         const int rr = rowok ? r : nrow - 1;
         y = y0 + rr;
         roff = rr * rowstride;
+        ylo = y0 + __builtin_amdgcn_readfirstlane(rr);
+        yhi = y0 + __builtin_amdgcn_readlane(rr, 63);
     }
     const bool act0 = rowok && x0 < g.nx, act1 = rowok && x0 + 1 < g.nx;
     const int xc = x0 < g.nx ? x0 : ((g.nx - 1) & ~1);
@@ -472,8 +475,7 @@ __device__ __forceinline__ void adjoint_pair(const AdjArgs &a, int b, int nb, do
         ok0[f] = (unsigned)(x0 + dx) < (unsigned)g.nx;
         ok1[f] = (unsigned)(x0 + 1 + dx) < (unsigned)g.nx;
     }
-    const int last_row = y0 + (ROWS ? g.rows : 1) - 1;
-    if (z >= 1 && z <= g.nz - 2 && y0 >= 1 && last_row <= g.ny - 2) {
+    if (z >= 1 && z <= g.nz - 2 && ylo >= 1 && yhi <= g.ny - 2) {
         const float2 *own = mu1 + own_line;
 #pragma unroll
         for (int f = 0; f < DEG; ++f) {

@@ -153,13 +153,14 @@ __device__ __forceinline__ RowCoord decode_rows(const Geometry &g, int b, int nb
     return rc;
 }
 
+// First element of the x-line (z, y, v, c) in either layout; offsets within a z plane are int.
 __device__ __forceinline__ long long line_index(const Geometry &g, int c, int z, int y, int v)
 {
-    return ((long long)((z * g.ny + y) * g.V + v) * g.C + c) * g.lx;
+    return (long long)z * g.zs + (y * (int)g.ys + v * (int)g.vs + c * (int)g.cs);
 }
 
 // Elements between the same x of consecutive components of one line.
-__device__ __forceinline__ long long comp_stride(const Geometry &g) { return g.lx; }
+__device__ __forceinline__ long long comp_stride(const Geometry &g) { return g.cs; }
 
 // ------------------------------------------------------------------ local Hamiltonian --
 

@@ -268,7 +268,11 @@ bool finish_geometry(Geometry &g)
     // rows of Z's 34 pairs (93%, tiles of 14) 5%; the tile heights with such divisors are
     // autotuner candidates.
     int rows = 1;
-    if (g.xpl == 2) {
+    if (g.xpl == 2 && g.slabs && g.ty > 1 && ((g.ty * g.lx) % 16 == 0 || g.ty >= g.ny)) {
+        // slab layout, tiles of rows holding a multiple of 16 values (or one tile): the tile is
+        // the group, one contiguous run of pairs from a 128-byte boundary
+        rows = g.ty;
+    } else if (g.xpl == 2) {
         const int want = g.nx <= 32 ? 4 : (g.nx <= 64 ? 2 : 1);
         while (rows < want && g.ty % (2 * rows) == 0) rows *= 2;
         const double floor_use = g.V <= 12 ? 0.0 : 0.9;
@@ -316,6 +320,21 @@ Geometry base_geometry(const gpe_config &cfg)
     const long long align = cfg.precision == GPE_FP64 ? 8 : 16;
     g.lx = ((long long)cfg.nx + align - 1) / align * align;
     if (20 * g.lx > 21 * (long long)cfg.nx) g.lx = ((long long)cfg.nx + 1) / 2 * 2;
+    // FP32 lines that are not whole 128-byte lines: component slabs of contiguous rows, whose
+    // row groups start on 128-byte boundaries (finish_geometry)
+    g.slabs = cfg.precision != GPE_FP64 && g.lx % align != 0;
+    if (g.slabs) {
+        const long long ls = ((long long)cfg.ny * g.lx + align - 1) / align * align;
+        g.ys = g.lx;
+        g.cs = ls;
+        g.vs = (long long)cfg.ncomp * ls;
+        g.zs = g.vs * g.V;
+    } else {
+        g.cs = g.lx;
+        g.vs = (long long)cfg.ncomp * g.lx;
+        g.ys = g.vs * g.V;
+        g.zs = g.ys * cfg.ny;
+    }
     g.xcd_remap = 1;
     return g;
 }
@@ -407,13 +426,14 @@ int gpe_create(const gpe_config *cfg, gpe_ctx **out)
         delete c;
         return GPE_EINVAL;
     }
-    // 32-bit index arithmetic in the kernels: z-plane offsets ((ny + 2) V C lx elements, the
-    // per-face line offsets and row strides) and line numbers ((z ny + y) V + v, line_index)
-    if ((long long)(g.ny + 2) * g.V * g.C * g.lx >= (1ll << 31) || (long long)g.nz * g.ny * g.V >= (1ll << 31)) {
+    // 32-bit index arithmetic in the kernels: offsets within a z plane and to its neighbours
+    // (line_index, the per-face line offsets, row strides: below 4 zs) and line numbers
+    // ((z ny + y) V + v)
+    if (4 * g.zs >= (1ll << 31) || (long long)g.nz * g.ny * g.V >= (1ll << 31)) {
         delete c;
         return GPE_EINVAL;
     }
-    c->level_elems = (long long)cfg->ncomp * g.nz * g.ny * g.V * g.lx;
+    c->level_elems = g.nz * g.zs;
 
     Physics &ph = c->ph;
     for (int k = 0; k < 3; ++k) ph.cell[k] = cfg->block_scale * c->lat->block_width[k];
@@ -456,8 +476,8 @@ int gpe_create(const gpe_config *cfg, gpe_ctx **out)
             const int pk = c->lat->face[v * deg + f];
             const long long dy = ((pk >> 2) & 3) - 1, dz = ((pk >> 4) & 3) - 1, t = pk >> 8;
             facex[2 * (v * deg + f)] = pk;
-            // |offset| < (ny + 2) V C lx, checked against the int range at creation
-            facex[2 * (v * deg + f) + 1] = (int)(((dz * g.ny + dy) * V + (t - v)) * g.C * g.lx);
+            // |offset| < 3 zs, checked against the int range at creation
+            facex[2 * (v * deg + f) + 1] = (int)(dz * g.zs + dy * g.ys + (t - v) * g.vs);
         }
         int nf = deg;  // pad faces (zero Hodge weight) trail the real ones
         while (nf > 0 && c->lat->hodge[v * deg + nf - 1] == 0.0) --nf;
@@ -1694,6 +1714,11 @@ int gpe_plan_traversal(const gpe_config *cfg, int tile_rows, int xcd_remap, int 
     out->ntile = g.ntile;
     out->xcd_remap = g.xcd_remap | (g.v_inner << 1);
     out->lx = g.lx;
+    out->slabs = g.slabs;
+    out->ys = g.ys;
+    out->vs = g.vs;
+    out->cs = g.cs;
+    out->zs = g.zs;
     out->waves = g.waves;
     out->blocks = g.blocks;
     const FastDiv *d[5] = {&g.f_nseg, &g.f_V, &g.f_ty, &g.f_nz, &g.f_pairs};

@@ -18,24 +18,31 @@ struct FastDiv {
     unsigned d, m, l;
 };
 
-// Device layout of one time level: z, y, unknown v, component, and x fastest,
-//   element(z, y, v, c, x) = ((((z * ny + y) * V + v) * C + c) * lx + x
+// Device layout of one time level, in one of two orders (Geometry::slabs):
+//   lines (0):  element(z, y, v, c, x) = ((((z * ny + y) * V + v) * C + c) * lx + x
+//   slabs (1):  element(z, v, c, y, x) = ((z * V + v) * C + c) * ls + y * lx + x
 // with lx >= nx rounded up so every x-line starts on a 128-byte boundary (or, where that
-// would pad a line by more than 1/20, lx = nx rounded up to even).  One wave
-// handles 64 consecutive x of one (z, y, v) line, so the unknown v, the face table row and
-// the neighbour rows are uniform across the wave and every neighbour load is a contiguous
-// 64-lane access.  Out-of-box neighbours read as zero (Dirichlet, as the reference's
-// never-written halo).
-// The C components of a line are adjacent (C consecutive x-lines): a wave's own, previous
-// and next values are each one contiguous run of C line segments, which streams ~12% faster
-// from HBM than C separate component planes (measured with the sweep's read-2-write-1 pattern).
+// would pad a line by more than 1/20, lx = nx rounded up to even), and ls = ny lx rounded up
+// to 128 bytes.  Both are addressed through the strides ys, vs, cs, zs (line_index).
+// Lines: the C components of a line are adjacent (C consecutive x-lines), so a wave's own,
+// previous and next values are each one contiguous run of C line segments, which streams
+// ~12% faster from HBM than C separate component planes (measured with the sweep's
+// read-2-write-1 pattern).  Slabs (FP32 contexts whose lines are not whole 128-byte lines):
+// the rows of one (z, v, c) follow each other, so x pairs spread over a tile of rows are one
+// contiguous run that starts on a 128-byte boundary when the tile's rows hold a multiple of
+// 16 values -- lines of 82 or 58 values would otherwise start inside a cache line and every
+// quarter-wave access would touch three lines instead of two.  One wave handles 64
+// consecutive x (or x pairs) of one (z, y, v) line or of a group of rows, so the unknown v
+// and the face table row are uniform across the wave.  Out-of-box neighbours read as zero
+// (Dirichlet, as the reference's never-written halo).
 struct Geometry {
     int nx, ny, nz, V, deg;
     int C;             // complex components (1, 3, 5)
     FastDiv f_nseg, f_V, f_ty, f_nz;  // the decode's divisors (f_ty divides by tyr)
     FastDiv f_pairs;   // divides a lane's pair slot by `pairs` (rows > 1)
     int xpl;           // consecutive x values per lane (1, or 2 for the FP32 sweep)
-    int rows;          // y rows of a row group (1; 2-8: x pairs spread over the group's rows)
+    int rows;          // y rows of a row group (1; 2-8, or the tile's rows in the slab layout
+                       // when they start on 128-byte boundaries: x pairs spread over the rows)
     int pairs;         // x pairs per row: ceil(nx / 2)
     int nseg;          // waves per x-line (rows == 1: ceil(nx / (64 xpl))) or per row group
                        // (rows > 1: ceil(rows * pairs / 64))
@@ -43,6 +50,9 @@ struct Geometry {
     int tyr;           // row groups per tile: ty / rows
     int ntile;         // ceil(ny / ty)
     long long lx;      // elements per x-line
+    int slabs;         // layout: 0 x-lines in (z, y, v, c) order, 1 component slabs of rows
+    long long ys, vs, cs, zs;  // elements between rows, unknowns, components and z planes
+                               // (4 zs < 2^31: offsets within a plane and its neighbours are int)
     long long waves;   // total waves of a sweep
     int blocks;        // workgroups of a sweep (4 waves each)
     int xcd_remap;     // give each XCD a contiguous range of the traversal order
@@ -81,7 +91,7 @@ struct Tables {
     const double *dv;      // V: s^3 * dual volume of unknown v
     const int *nface;      // V: faces of unknown v before its zero-weight pad faces
     const int *facex;      // V*deg pairs (packed face, elements from unknown v's own line to the
-                           // face's neighbour line: ((dz ny + dy) V + t - v) C lx, y / z inside)
+                           // face's neighbour line: dz zs + dy ys + (t - v) vs, y / z inside)
 };
 
 // FP32 copies of the physics constants, converted once on the host for the FP32 sweep.

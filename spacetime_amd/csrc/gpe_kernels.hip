@@ -287,13 +287,18 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
         // the lane's row: y0 (+ r under ROWS, clamped into the box) at element offset roff
         bool rowok = true;
         int y = y0, roff = 0;
-        const int rowstride = g.V * g.C * (int)g.lx;  // elements between rows y and y + 1
+        const int rowstride = (int)g.ys;  // elements between rows y and y + 1
+        int ylo = y0, yhi = y0;           // the rows of the wave's lanes
         if constexpr (ROWS) {
             const int nrow = g.ny - y0 < g.rows ? g.ny - y0 : g.rows;  // the group's rows in the box
             rowok = r < nrow;
             const int rr = rowok ? r : nrow - 1;
             y = y0 + rr;
             roff = rr * rowstride;
+            // a lane's row grows with its lane number: lanes 0 and 63 bound the wave's rows
+            const int r0 = __builtin_amdgcn_readfirstlane(rr), r63 = __builtin_amdgcn_readlane(rr, 63);
+            ylo = y0 + r0;
+            yhi = y0 + r63;
         }
         const bool act0 = rowok && x0 < g.nx, act1 = rowok && x0 + 1 < g.nx;
         const int xc = x0 < g.nx ? x0 : ((g.nx - 1) & ~1);  // even, so xc + 1 <= lx - 1
@@ -353,8 +358,7 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
         }
         // a wave whose rows and plane are all inside [1, n - 2] has every y / z neighbour inside
         // the box: its neighbour lines are the own line plus the per-face offsets of the table
-        const int last_row = y0 + (ROWS ? g.rows : 1) - 1;
-        if (z >= 1 && z <= g.nz - 2 && y0 >= 1 && last_row <= g.ny - 2) {
+        if (z >= 1 && z <= g.nz - 2 && ylo >= 1 && yhi <= g.ny - 2) {
             const float2 *own = cur + own_line;
 #pragma unroll
             for (int f = 0; f < DEG; ++f) {

@@ -107,9 +107,16 @@ def test_wide_rows_span_several_waves(oracle, precision, tol, xpl, nx, flags):
     ("bcc", 1, (192, 6, 3), (4, 3, 2)),  # 2-row groups of 96 pairs over 3 waves; tile 2 holds rows 4, 5
     ("a15", 5, (82, 7, 2), (6, 2, 2)),   # 3-row groups of 41 pairs over 2 waves, no XCD remap
     ("z", 3, (68, 16, 2), (14, 3, 2)),   # 7-row groups of 34 pairs over 4 waves
+    # slab layout, the tile as the group: one run of pairs from a 128-byte boundary
+    ("a15", 5, (82, 13, 2), (8, 1, 2)),  # 8 rows of 41 pairs over 6 waves; tile 2 holds 5 rows
+    ("c15", 3, (57, 11, 2), (8, 3, 2)),  # 8 rows of 29 pairs over 4 waves, v inside; tile 2 holds 3 rows
+    ("z", 5, (68, 10, 3), (12, 1, 2)),   # one tile of all 10 rows
+    ("cube", 1, (90, 20, 3), (24, 0, 2)),  # one tile of 20 rows, no XCD remap
+    ("fcc", 5, (45, 17, 3), (16, 1, 2)),  # degree 8, 16 rows of 23 pairs; tile 2 holds 1 row
 ])
 def test_packed_rows_match_oracle(oracle, lattice_name, ncomp, dims, traversal):
-    """x pairs spread over groups of 2-8 y rows (lines that one row per wave leaves partly idle)."""
+    """x pairs spread over groups of 2-8 y rows or over whole tiles (lines that one row per wave
+    leaves partly idle)."""
     cfg = _config(lattice_name, ncomp, gpe.FP32, dims)
     (g0, g1), (o0, o1) = _run_both(oracle, cfg, 10, field=FIELD if ncomp > 1 else None, traversal=traversal)
     assert rel_l2(g0, o0) < 2e-5

@@ -4,7 +4,9 @@ library's own traversal plan (gpe_plan_traversal, computed by the same host code
 launches).  For every wave of a sweep the unknown, plane, rows and x positions it touches lie
 inside the box or are masked, the fast divisions agree with exact integer division, the waves
 cover every (z, y, v, x) of the box exactly once, and both workgroup orders of consecutive
-sweeps (the XCD remap and its reversal) are permutations of the launch."""
+sweeps (the XCD remap and its reversal) are permutations of the launch.  The level layout's
+strides place every element of the box at its own address inside the level, and x pairs spread
+over a tile of the slab layout load from 128-byte boundaries."""
 import ctypes
 
 import numpy as np
@@ -19,7 +21,8 @@ class _Plan(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in ("nx", "ny", "nz", "V", "C", "x_per_lane", "rows", "pairs", "nseg",
                                              "tile_rows", "row_groups", "ntile", "xcd_remap")] + [
         ("lx", ctypes.c_longlong), ("waves", ctypes.c_longlong), ("blocks", ctypes.c_int)] + [
-        (n, ctypes.c_uint * 3) for n in ("div_nseg", "div_V", "div_rows", "div_nz", "div_pairs")]
+        (n, ctypes.c_uint * 3) for n in ("div_nseg", "div_V", "div_rows", "div_nz", "div_pairs")] + [
+        ("slabs", ctypes.c_int)] + [(n, ctypes.c_longlong) for n in ("ys", "vs", "cs", "zs")]
 
 
 def _plan(cfg, ty, remap, xpl):
@@ -174,25 +177,62 @@ def test_decode_stays_in_range_and_covers_the_box(lat, ncomp, precision, dims):
                     # an x pair that starts inside the line ends inside its padding at most
                     assert np.all(x[live] + 1 < p.lx)
                 assert count.min() == 1 and count.max() == 1, (ty, remap, xpl)
+                if p.rows > 1 and p.slabs and p.rows == p.tile_rows and (p.rows * p.lx % 16 == 0 or p.ntile == 1):
+                    # the tile is one run of pairs: every wave's first lane starts a 128-byte line
+                    first = (z * p.zs + v * p.vs + y0 * p.ys)[valid] + (r[valid, 0] * p.ys + x[valid, 0])
+                    assert np.all(first % 16 == 0), (ty, remap)
                 checked += 1
     assert checked >= 32
 
 
+@pytest.mark.parametrize("lat,ncomp,precision,dims", CASES + [("c15", 5, gpe.FP32, (57, 9, 3))])
+def test_layout_addresses_every_element_once_inside_the_level(lat, ncomp, precision, dims):
+    """z zs + y ys + v vs + c cs + x over the box: distinct, inside the level (nz zs elements),
+    offsets within a plane and its neighbours in int range; slabs exactly for FP32 lines that
+    are not whole 128-byte lines, with contiguous rows."""
+    cfg = gpe.Config(lattice=lat, ncomp=ncomp, precision=precision, dims=dims)
+    nx, ny, nz = dims
+    rc, p = _plan(cfg, 1, 1, 1)
+    assert rc == 0
+    align = 8 if precision == gpe.FP64 else 16
+    assert p.slabs == (precision == gpe.FP32 and p.lx % align != 0)
+    if p.slabs:
+        assert p.ys == p.lx and p.cs % align == 0 and p.cs >= ny * p.lx and p.lx == nx + nx % 2
+    else:
+        assert p.cs == p.lx and p.ys == p.V * ncomp * p.lx
+    z, y, v, c, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(p.V), np.arange(ncomp), np.arange(nx),
+                                indexing="ij")
+    addr = z * p.zs + y * p.ys + v * p.vs + c * p.cs + x
+    assert addr.min() == 0 and addr.max() < nz * p.zs
+    assert np.unique(addr).size == addr.size
+    assert 4 * p.zs < 2 ** 31
+
+
 @pytest.mark.parametrize("lat,ncomp,dims,ty,rows", [
     ("a15", 5, (82, 13, 2), 12, 3),  # 41 pairs: 3 rows fill 96% of two waves
-    ("a15", 5, (82, 13, 2), 8, 1),   # 4 or 8 rows fill 85%: below the large-cell floor of 90%
+    ("a15", 5, (82, 13, 2), 4, 1),   # 4 rows fill 85%: below the large-cell floor of 90%
+    ("a15", 5, (82, 13, 2), 8, 8),   # slabs: 8 rows of 82 values start on 128 bytes, the tile is the group
+    ("a15", 5, (82, 13, 2), 16, 13), # one tile
     ("z", 5, (68, 15, 2), 14, 7),    # 34 pairs: 7 rows fill 93% of four waves
-    ("z", 5, (68, 15, 2), 8, 1),
+    ("z", 5, (68, 15, 2), 8, 8),
+    ("z", 5, (68, 15, 2), 6, 1),
     ("bcc", 3, (70, 9, 2), 7, 7),    # 35 pairs: 7 rows fill 96%
-    ("bcc", 3, (70, 9, 2), 8, 8),    # small cells take the best power of two (88%)
+    ("bcc", 3, (70, 9, 2), 6, 3),    # small cells take the best group of 1-8 rows (82%)
+    ("bcc", 3, (70, 9, 2), 8, 8),
     ("c15", 1, (20, 7, 2), 6, 6),    # 10 pairs: 6 rows fill 94% of one wave
-    ("c15", 5, (57, 9, 2), 8, 2),    # lines of <= 64 values: 2 rows
+    ("c15", 5, (57, 9, 2), 8, 8),
+    ("c15", 5, (57, 9, 2), 2, 2),    # lines of <= 64 values: 2 rows
+    ("cube", 5, (90, 40, 2), 24, 24),
+    ("cube", 5, (90, 40, 2), 12, 4),
+    ("cube", 5, (293, 40, 2), 24, 3), # 293 values in lines of 304 (3.7% padding): line layout
+    ("bcc", 3, (128, 40, 2), 16, 1), # whole 128-byte lines keep the line layout: one row per wave
 ])
 def test_row_group_choice(lat, ncomp, dims, ty, rows):
-    """The row group a tile height gets, and that the cases above cover groups of 3, 6 and 7."""
+    """The row group a tile height gets, and that the cases above cover groups of 3, 6 and 7 and
+    tiles as groups."""
     cfg = gpe.Config(lattice=lat, ncomp=ncomp, precision=gpe.FP32, dims=dims)
     rc, p = _plan(cfg, ty, 1, 2)
-    assert rc == 0 and p.rows == rows and p.tile_rows == ty
+    assert rc == 0 and p.rows == rows and p.tile_rows == min(ty, dims[1])
 
 
 @pytest.mark.parametrize("nb", [1, 7, 8, 9, 63, 64, 1000, 4097])
