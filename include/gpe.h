@@ -10,8 +10,9 @@
  * State layout across the ABI (host or device arrays, complex interleaved re/im):
  *   psi[2 * (((c * nz + z) * ny + y) * nx + x) * V + v) + {0, 1}]
  * component c (m = F, F-1, ..., -F), block (x, y, z), unknown v of the block.  The device
- * keeps its own layout (x-fastest lines per (component, z, y, unknown); gpe_internal.h);
- * the ABI layout is independent of it.
+ * keeps its own layout (x-fastest lines per (component, z, y, unknown), or for FP32 short
+ * lines component slabs of contiguous rows; gpe_internal.h, gpe_plan_traversal); the ABI
+ * layout is independent of it.
  *
  * Threading: a context owns one HIP stream; calls on one context are stream-ordered and
  * not thread-safe; distinct contexts may be driven from distinct threads.  Calls that
@@ -269,7 +270,8 @@ int gpe_traversal(const gpe_ctx *ctx, int *tile_rows, int *xcd_remap, int *x_per
 
 /* The launch a sweep of a configuration would make for a traversal, computed on the host
  * without a device: workgroups of four waves, each wave a 64-lane segment of one (z, y, v)
- * x-line (rows = 1) or of a group of `rows` y rows (x pairs spread over the rows), in the
+ * x-line (rows = 1) or of a group of `rows` y rows (x pairs spread over the rows; in the slab
+ * layout a tile whose rows hold a multiple of 16 values is one group), in the
  * order y-tile, z, row (group) in tile, unknown v, segment (or segment, v: flag bit 1; the
  * plan's xcd_remap holds the flags); and the constants of the wave
  * decode's divisions by invariants (d, m, l: n / d = (t + n) >> l for n < 2^31, t the high
