@@ -284,8 +284,9 @@ typedef struct {
     int blocks;
     unsigned div_nseg[3], div_V[3], div_rows[3], div_nz[3], div_pairs[3]; /* d, m, l */
     /* level layout: element (z, y, v, c, x) at z zs + y ys + v vs + c cs + x; slabs = 1 when
-     * the rows of one (z, v, c) are contiguous (FP32 lines that are not whole 128-byte lines),
-     * 0 when the C components of one (z, y, v) are adjacent x-lines */
+     * the rows of one (z, v, c) are contiguous (FP32 lines whose even length is not a whole
+     * number of 128-byte lines; lx = that even length), 0 when the C components of one
+     * (z, y, v) are adjacent x-lines */
     int slabs;
     long long ys, vs, cs, zs;
 } gpe_traversal_plan;

@@ -21,13 +21,15 @@ struct FastDiv {
 // Device layout of one time level, in one of two orders (Geometry::slabs):
 //   lines (0):  element(z, y, v, c, x) = ((((z * ny + y) * V + v) * C + c) * lx + x
 //   slabs (1):  element(z, v, c, y, x) = ((z * V + v) * C + c) * ls + y * lx + x
-// with lx >= nx rounded up so every x-line starts on a 128-byte boundary (or, where that
-// would pad a line by more than 1/20, lx = nx rounded up to even), and ls = ny lx rounded up
-// to 128 bytes.  Both are addressed through the strides ys, vs, cs, zs (line_index).
+// x-lines (FP64, and FP32 lines of a multiple of 16 values once rounded up to even): lx >= nx
+// rounded up so every x-line starts on a 128-byte boundary (or, where that would pad a line by
+// more than 1/20, lx = nx rounded up to even).  Slabs (the other FP32 lines): lx = nx rounded
+// up to even, ls = ny lx rounded up to 128 bytes.  Both are addressed through the strides ys,
+// vs, cs, zs (line_index).
 // Lines: the C components of a line are adjacent (C consecutive x-lines), so a wave's own,
 // previous and next values are each one contiguous run of C line segments, which streams
 // ~12% faster from HBM than C separate component planes (measured with the sweep's
-// read-2-write-1 pattern).  Slabs (FP32 contexts whose lines are not whole 128-byte lines):
+// read-2-write-1 pattern).  Slabs:
 // the rows of one (z, v, c) follow each other, so x pairs spread over a tile of rows are one
 // contiguous run that starts on a 128-byte boundary when the tile's rows hold a multiple of
 // 16 values -- lines of 82 or 58 values would otherwise start inside a cache line and every
