@@ -387,13 +387,14 @@ __device__ __forceinline__ void sweep_body_pair(const SweepArgs &a, int sweep, i
             }
         }
         // degree-8 tables (FCC) pad the 4-face unknowns with 4 zero-weight faces: skip their
-        // loads.  With 5 components and one row per wave the degree-8 faces go in two batches
+        // loads.  With 5 components the degree-8 faces go in two batches
         // of 4, so a wave holds one batch of neighbour pairs at a time (the second batch, loaded
         // after the first is summed and only by 8-face waves, cannot be hoisted past the
         // wave-uniform test): 159 instead of 228 VGPRs, 3 waves per SIMD instead of 2, +6-12% on
-        // FCC boxes of 96-128 blocks.  Fewer components or packed rows gain nothing from it.
+        // FCC boxes of 96-128 blocks; pairs over row groups of slab rows +1-1.5% on FCC 45-100^3.
+        // Fewer components gain nothing from it.
         const int nf = DEG == 8 ? a.tb.nface[v] : DEG;
-        constexpr int FB = (DEG == 8 && C == 5 && !ROWS) ? 4 : DEG;
+        constexpr int FB = (DEG == 8 && C == 5) ? 4 : DEG;
 #pragma unroll
         for (int f0 = 0; f0 < DEG; f0 += FB) {
             if (f0 > 0 && f0 >= nf) break;
