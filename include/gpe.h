@@ -250,7 +250,7 @@ int gpe_ensemble_timing(const gpe_ensemble *ens, double *ms_per_sweep, double *u
 
 /* -------------------------------------------------------------------- autotuning ---- */
 /* Time the traversal candidates (y-tile rows 1..32, and 3..28 where they give row groups that
- * are not powers of two, x the order flags of gpe_set_traversal -- the second only where lines
+ * are not powers of two, and the whole plane of ny <= 1024 rows, x the order flags of gpe_set_traversal -- the second only where lines
  * span several waves -- x one or two x values per lane, the latter for FP32) on scratch copies
  * of the context's two levels for trial_sweeps sweeps each, and keep the fastest.  Results are cached per
  * (lattice, ncomp, precision, nx, ny, nz) for the process; pass trial_sweeps = 0 to apply a
@@ -259,8 +259,9 @@ int gpe_ensemble_timing(const gpe_ensemble *ens, double *ms_per_sweep, double *u
 int gpe_autotune(gpe_ctx *ctx, int trial_sweeps, int *tile_rows, int *xcd_remap, int *x_per_lane,
                  double *ms_per_sweep);
 
-/* Set / read the sweep traversal directly: y-tile rows (1..64), order flags (0..3: bit 0 gives
- * each XCD a contiguous range of the traversal, bit 1 puts the unknown v inside the x segment
+/* Set / read the sweep traversal directly: y-tile rows (1..1024; ny or more: one tile of the
+ * whole plane), order flags (0..3: bit 0 gives each XCD a contiguous range of the traversal,
+ * bit 1 puts the unknown v inside the x segment
  * so that a workgroup's waves take neighbouring unknowns of one stretch of x; the second
  * matters only where lines span several waves), x values per lane (1, or 2 for FP32).  No
  * counterpart in the reference (fixed thread blocks, Spin1GpeBCC.cu:39-41).  Results do not

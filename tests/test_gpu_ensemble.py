@@ -112,7 +112,7 @@ def test_autotune_keeps_results_identical():
         st.set_state(psi)
         st.start(gpe.START_EULER)
         t = st.autotune(5)
-        # the autotuner's tile heights: powers of two and multiples of 3, 5, 6 and 7 up to 32
+        # the autotuner's tile heights: powers of two and multiples of 3, 5, 6 and 7 up to 32 (the plane, 24 rows, is one of them)
         assert t["tile_rows"] in (1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32)
         assert t["xcd_remap"] in (0, 1, 2, 3) and t["ms_per_sweep"] > 0  # order flags
         assert t["x_per_lane"] in (1, 2)

@@ -113,6 +113,7 @@ def test_wide_rows_span_several_waves(oracle, precision, tol, xpl, nx, flags):
     ("z", 5, (68, 10, 3), (12, 1, 2)),   # one tile of all 10 rows
     ("cube", 1, (90, 20, 3), (24, 0, 2)),  # one tile of 20 rows, no XCD remap
     ("fcc", 5, (45, 17, 3), (16, 1, 2)),  # degree 8, 16 rows of 23 pairs; tile 2 holds 1 row
+    ("a15", 1, (82, 41, 2), (41, 0, 2)),  # the whole plane: 41 rows of 41 pairs over 27 waves
 ])
 def test_packed_rows_match_oracle(oracle, lattice_name, ncomp, dims, traversal):
     """x pairs spread over groups of 2-8 y rows or over whole tiles (lines that one row per wave
