@@ -134,6 +134,8 @@ CASES = [  # lattice, ncomp, precision, dims
     ("bcc", 1, gpe.FP32, (70, 9, 2)), ("c15", 3, gpe.FP32, (20, 7, 2)),
     # one component over long lines: several waves per line
     ("cube", 1, gpe.FP32, (293, 4, 2)), ("bcc", 1, gpe.FP32, (600, 3, 2)), ("z", 1, gpe.FP32, (131, 5, 2)),
+    # planes of more than 64 rows (whole-plane tiles of 82 and 130 rows)
+    ("a15", 1, gpe.FP32, (82, 82, 1)), ("cube", 3, gpe.FP32, (58, 130, 1)),
 ]
 
 
@@ -144,7 +146,7 @@ def test_decode_stays_in_range_and_covers_the_box(lat, ncomp, precision, dims):
     xpls = (1, 2) if precision == gpe.FP32 else (1,)
     checked = 0
     for xpl in xpls:
-        for ty in (1, 2, 3, 4, 5, 6, 7, 8, 12, 14, 16, 32, 64):
+        for ty in (1, 2, 3, 4, 5, 6, 7, 8, 12, 14, 16, 32, 64, 1024):  # 1024: the whole plane
             for remap in (0, 1, 2, 3):
                 rc, p = _plan(cfg, ty, remap, xpl)
                 assert rc == 0
@@ -266,7 +268,9 @@ def test_plan_rejects_bad_arguments():
     cfg = gpe.Config(lattice="bcc", ncomp=3, precision=gpe.FP64, dims=(8, 8, 8))
     assert _plan(cfg, 8, 1, 2)[0] == -22  # FP64 has one x per lane
     assert _plan(gpe.Config(lattice="bcc", ncomp=1, dims=(8, 8, 8)), 8, 1, 4)[0] == -22
-    assert _plan(cfg, 0, 1, 1)[0] == -22
+    assert _plan(cfg, 0, 1, 1)[0] == -22 and _plan(cfg, 1025, 1, 1)[0] == -22  # tile rows 1..1024
+    rc, p = _plan(cfg, 1024, 1, 1)
+    assert rc == 0 and p.tile_rows == 8 and p.ntile == 1  # clamped to the plane
     assert _plan(gpe.Config(lattice="bcc", dims=(0, 8, 8)), 8, 1, 1)[0] == -22
     assert _plan(cfg, 8, 4, 1)[0] == -22 and _plan(cfg, 8, -1, 1)[0] == -22  # order flags 0..3
     rc, p = _plan(cfg, 8, 3, 1)
