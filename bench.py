@@ -346,6 +346,7 @@ def run_lattices(args, log):
         nodes = float(np.prod(dims) * lat.V)
         gbs = nb / (ms * 1e-3) / 1e9
         cases.append(dict(lattice=name, components=ncomp, blocks=list(dims), unknowns=nodes, dt=dt,
+                          layout=gpe.level_layout(cfg)["layout"],
                           traversal=[tune["tile_rows"], tune["xcd_remap"], tune["x_per_lane"]],
                           kernel_ms_per_sweep=ms, node_updates_per_s=nodes * args.steps / wall,
                           achieved_gbs=gbs, frac=gbs / PEAK_HBM_GBS, norm=o.norm, finite=bool(o.finite),

@@ -79,6 +79,14 @@ class _AdjRes(ctypes.Structure):
     ]
 
 
+class _Plan(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("nx", "ny", "nz", "V", "C", "x_per_lane", "rows", "pairs", "nseg",
+                                             "tile_rows", "row_groups", "ntile", "xcd_remap")] + [
+        ("lx", ctypes.c_longlong), ("waves", ctypes.c_longlong), ("blocks", ctypes.c_int)] + [
+        (n, ctypes.c_uint * 3) for n in ("div_nseg", "div_V", "div_rows", "div_nz", "div_pairs")] + [
+        ("slabs", ctypes.c_int)] + [(n, ctypes.c_longlong) for n in ("ys", "vs", "cs", "zs")]
+
+
 class GpeError(RuntimeError):
     pass
 
@@ -125,6 +133,8 @@ def load_library(path=LIB_PATH):
                                  ctypes.POINTER(ctypes.c_int), _dp]
     lib.gpe_set_trap.argtypes = [vp, _dp]
     lib.gpe_stream_bandwidth.argtypes = [ctypes.c_longlong, ctypes.c_int, _dp, _dp]
+    lib.gpe_plan_traversal.argtypes = [ctypes.POINTER(_Config), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(_Plan)]
     lib.gpe_set_traversal.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.gpe_traversal.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_int)]
@@ -508,6 +518,18 @@ class Stepper:
         by = ctypes.c_double()
         self._check(self.lib.gpe_last_step_timing(self._h, ctypes.byref(ms), ctypes.byref(by)), "timing")
         return ms.value, by.value
+
+
+def level_layout(cfg):
+    """The device layout of a configuration's levels, computed on the host (gpe_plan_traversal):
+    'slabs' or 'lines', elements per x-line and the element strides of rows, unknowns, components
+    and z planes."""
+    lib = load_library()
+    p = _Plan()
+    rc = lib.gpe_plan_traversal(cfg.to_c(), 1, 1, 1, ctypes.byref(p))
+    if rc != 0:
+        raise GpeError("gpe_plan_traversal failed: %s (%d)" % (ERRORS.get(rc, "?"), rc))
+    return dict(layout="slabs" if p.slabs else "lines", lx=p.lx, ys=p.ys, vs=p.vs, cs=p.cs, zs=p.zs)
 
 
 def stream_bandwidth(nbytes=1 << 30, reps=20):

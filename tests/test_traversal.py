@@ -17,20 +17,13 @@ from spacetime_amd import gpe
 U32 = np.uint64(0xFFFFFFFF)
 
 
-class _Plan(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_int) for n in ("nx", "ny", "nz", "V", "C", "x_per_lane", "rows", "pairs", "nseg",
-                                             "tile_rows", "row_groups", "ntile", "xcd_remap")] + [
-        ("lx", ctypes.c_longlong), ("waves", ctypes.c_longlong), ("blocks", ctypes.c_int)] + [
-        (n, ctypes.c_uint * 3) for n in ("div_nseg", "div_V", "div_rows", "div_nz", "div_pairs")] + [
-        ("slabs", ctypes.c_int)] + [(n, ctypes.c_longlong) for n in ("ys", "vs", "cs", "zs")]
+_Plan = gpe._Plan  # the ctypes mirror of gpe_traversal_plan
 
 
 def _plan(cfg, ty, remap, xpl):
     lib = gpe.load_library()
-    f = lib.gpe_plan_traversal
-    f.argtypes = [ctypes.POINTER(gpe._Config), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_Plan)]
     p = _Plan()
-    rc = f(cfg.to_c(), ty, remap, xpl, ctypes.byref(p))
+    rc = lib.gpe_plan_traversal(cfg.to_c(), ty, remap, xpl, ctypes.byref(p))
     return rc, p
 
 
@@ -275,3 +268,14 @@ def test_plan_rejects_bad_arguments():
     assert _plan(cfg, 8, 4, 1)[0] == -22 and _plan(cfg, 8, -1, 1)[0] == -22  # order flags 0..3
     rc, p = _plan(cfg, 8, 3, 1)
     assert rc == 0 and p.xcd_remap == 3
+
+
+def test_level_layout_wrapper():
+    """gpe.level_layout: slabs for FP32 lines whose even length is not a whole number of 128-byte
+    lines, x-lines otherwise and for FP64; strides as the plan gives them."""
+    lay = gpe.level_layout(gpe.Config(lattice="a15", ncomp=5, precision=gpe.FP32, dims=(82, 82, 4)))
+    assert lay["layout"] == "slabs" and lay["lx"] == 82 and lay["ys"] == 82 and lay["cs"] % 16 == 0
+    assert lay["cs"] >= 82 * 82 and lay["vs"] == 5 * lay["cs"] and lay["zs"] == 46 * lay["vs"]  # A15: 46 unknowns a block
+    assert gpe.level_layout(gpe.Config(lattice="bcc", ncomp=3, precision=gpe.FP32, dims=(128, 8, 4)))["layout"] == "lines"
+    lay = gpe.level_layout(gpe.Config(lattice="a15", ncomp=5, precision=gpe.FP64, dims=(82, 82, 4)))
+    assert lay["layout"] == "lines" and lay["cs"] == lay["lx"] and lay["ys"] == 46 * 5 * lay["lx"]
